@@ -1,0 +1,106 @@
+"""Build the step libraries in-tree (formura_amd/_build/).
+
+Every program in `PROGRAMS` is compiled by the formura_amd compiler into a
+header and a `.hip` source, then by hipcc (gfx950) into `lib<name>.so`.
+Flags: -O3, -ffp-contract=off (no fused multiply-add, so the arithmetic is the
+reference C's statement for statement), denormals kept, packed-FP32 off.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from typing import Dict, List, Optional, Tuple
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "formura_amd")
+BUILD = os.path.join(PKG, "_build")
+CSRC = os.path.join(PKG, "csrc")
+EXAMPLES = os.path.join(ROOT, "examples")
+MPI_STUB = os.path.join(ROOT, "include", "formura_serial_mpi")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-Xarch_host", "-mcmodel=medium"]
+
+
+def _ex(name: str) -> str:
+    return os.path.join(EXAMPLES, name)
+
+
+# name -> (fmr, yaml, precision, params, nc overrides, march variants or None)
+PROGRAMS: Dict[str, tuple] = {
+    # Gray-Scott 3D, 64^3, FP64 (the reference's CPU-runnable case)
+    "gs3d64": (_ex("gray-scott-3d-64.fmr"), _ex("gray-scott-3d-64.yaml"), "faithful", (), None, None),
+    # heat 3D, 512^3, FP64 and FP32 bulk
+    "heat512": (_ex("heat-3d-512.fmr"), _ex("heat-3d-512.yaml"), "faithful", (), None, None),
+    "heat512_f32": (_ex("heat-3d-512.fmr"), _ex("heat-3d-512.yaml"), "fp32", (), None, None),
+    # Gray-Scott 3D, 512^3: FP64 oracle run on the GPU and FP32 bulk
+    "gs3d512": (_ex("gray-scott-3d-512.fmr"), _ex("gray-scott-3d-512.yaml"), "faithful", (), None, None),
+    "gs3d512_f32": (_ex("gray-scott-3d-512.fmr"), _ex("gray-scott-3d-512.yaml"), "fp32", (), None, None),
+    # small odd-shaped variants for parity tests against the CPU oracle
+    "heat_t48": (_ex("heat-3d-512.fmr"), _ex("heat-3d-512.yaml"), "faithful", (),
+                 {"intra_node_shape": [40, 48, 72], "monitor_interval": 12}, None),
+    "heat_t48_f32": (_ex("heat-3d-512.fmr"), _ex("heat-3d-512.yaml"), "fp32", (),
+                     {"intra_node_shape": [40, 48, 72], "monitor_interval": 12}, None),
+    "gs3d_t": (_ex("gray-scott-3d-64.fmr"), _ex("gray-scott-3d-64.yaml"), "faithful", (),
+               {"intra_node_shape": [36, 70, 130], "monitor_interval": 20}, None),
+    "gs2d": (_ex("gray-scott-2d.fmr"), _ex("gray-scott-2d.yaml"), "faithful", (), None, None),
+}
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(BUILD, f"lib{name}.so")
+
+
+def generate(name: str):
+    sys.path.insert(0, ROOT)
+    from formura_amd.compiler.driver import compile_files, write_hip
+    fmr, yaml_, precision, params, over, variants = PROGRAMS[name]
+    c = compile_files(name, fmr, yaml_, precision, params, over)
+    h, src = write_hip(c, BUILD, variants)
+    return c, h, src
+
+
+def _run(cmd: List[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout[-6000:]}")
+
+
+def compile_hip(src: str, out: str, extra: Optional[List[str]] = None) -> str:
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", CSRC, "-I", MPI_STUB, "-I", BUILD, "-I", INCLUDE] + (extra or []) + \
+        [src, "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_program(name: str) -> str:
+    _, _, src = generate(name)
+    return compile_hip(src, lib_path(name))
+
+
+def build_all(names=None, jobs: int = 8) -> Dict[str, str]:
+    os.makedirs(BUILD, exist_ok=True)
+    names = list(names or PROGRAMS)
+    for n in names:      # code generation is quick and serial
+        generate(n)
+    out = {}
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(compile_hip, os.path.join(BUILD, f"{n}.hip"), lib_path(n)): n for n in names}
+        for f in cf.as_completed(futs):
+            out[futs[f]] = f.result()
+    return out
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or None
+    for k, v in build_all(names).items():
+        print(k, v)

@@ -1,0 +1,66 @@
+"""Compiler pipeline: `.fmr` + `.yaml` -> header + HIP step library source.
+
+Stage order mirrors exe-src/formura.hs:34-84: parse (with the numerical
+config and the injected extents), desugar, OM graph, manifestation, backend.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence, Tuple
+
+from .abi import Layout, header_text, make_layout
+from .emit_hip import emit_hip
+from .manifest import MMProgram, gen_mm_program
+from .numerical_config import NumericalConfig
+from .omgraph import OMProgram, gen_om_program
+from .parser import parse_program
+from .pointir import PointIR, lower_step
+
+
+@dataclass
+class Compiled:
+    name: str
+    om: OMProgram
+    mm: MMProgram
+    ir: PointIR
+    layout: Layout
+    precision: str
+
+
+def compile_source(name: str, fmr_text: str, nc: NumericalConfig, precision: str = "faithful",
+                   params: Sequence[str] = ()) -> Compiled:
+    prog = parse_program(fmr_text, nc)
+    om = gen_om_program(prog)
+    mm = gen_mm_program(om)
+    ir = lower_step(om, precision=precision, params=params)
+    lay = make_layout(name, mm, ir)
+    return Compiled(name, om, mm, ir, lay, precision)
+
+
+def compile_files(name: str, fmr_path: str, yaml_path: str, precision: str = "faithful",
+                  params: Sequence[str] = (), nc_override: Optional[dict] = None) -> Compiled:
+    with open(fmr_path, encoding="utf-8") as f:
+        text = f.read()
+    nc = NumericalConfig.from_file(yaml_path)
+    for k, v in (nc_override or {}).items():
+        setattr(nc, k, v)
+    return compile_source(name, text, nc, precision, params)
+
+
+def write_hip(c: Compiled, outdir: str, march_variants=None) -> Tuple[str, str]:
+    os.makedirs(outdir, exist_ok=True)
+    h = os.path.join(outdir, f"{c.name}.h")
+    src = os.path.join(outdir, f"{c.name}.hip")
+    _write_if_changed(h, header_text(c.layout))
+    _write_if_changed(src, emit_hip(c.layout, c.ir, c.precision, march_variants))
+    return h, src
+
+
+def _write_if_changed(path: str, text: str) -> None:
+    if os.path.exists(path):
+        with open(path, encoding="utf-8") as f:
+            if f.read() == text:
+                return
+    with open(path, "w", encoding="utf-8") as f:
+        f.write(text)

@@ -1,0 +1,272 @@
+// fmr_library.hpp -- host side of a generated step library: device state,
+// kernel variants, the upload / advance / download cycle behind
+// `Formura_Forward`, FP64 reductions and the extra device-resident entry
+// points declared in include/formura_amd.h.
+#pragma once
+
+#include "fmr_stencil.hpp"
+
+#include <vector>
+
+namespace fmr {
+
+// ---- FP64 reductions over a device field (deterministic two-pass) ---------
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// block b sums elements i = b*256 + t + k*(1024*256) in a fixed order
+template <class A, class B, int MODE>
+__global__ void __launch_bounds__(RED_THREADS) reduce_pass1(const A* a, const B* b, size_t n, double* part) {
+  __shared__ double s0[RED_THREADS / 64], s1[RED_THREADS / 64], s2[RED_THREADS / 64];
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  for (size_t i = blockIdx.x * (size_t)RED_THREADS + threadIdx.x; i < n; i += (size_t)RED_BLOCKS * RED_THREADS) {
+    const double x = (double)a[i];
+    if (MODE == 0) {   // sum and sum of squares
+      acc0 += x;
+      acc1 += x * x;
+    } else {           // L2 of difference, L2 of reference, max abs difference
+      const double r = (double)b[i];
+      const double d = x - r;
+      acc0 += d * d;
+      acc1 += r * r;
+      acc2 = fmax(acc2, fabs(d));
+    }
+  }
+  acc0 = wave_sum(acc0);
+  acc1 = wave_sum(acc1);
+  acc2 = wave_max(acc2);
+  const int wv = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) { s0[wv] = acc0; s1[wv] = acc1; s2[wv] = acc2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int k = 0; k < RED_THREADS / 64; ++k) { t0 += s0[k]; t1 += s1[k]; t2 = fmax(t2, s2[k]); }
+    part[3 * blockIdx.x + 0] = t0;
+    part[3 * blockIdx.x + 1] = t1;
+    part[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+__global__ void __launch_bounds__(64) reduce_pass2(const double* part, double* res) {
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  for (int k = threadIdx.x; k < RED_BLOCKS; k += 64) {
+    t0 += part[3 * k];
+    t1 += part[3 * k + 1];
+    t2 = fmax(t2, part[3 * k + 2]);
+  }
+  t0 = wave_sum(t0);
+  t1 = wave_sum(t1);
+  t2 = wave_max(t2);
+  if (threadIdx.x == 0) { res[0] = t0; res[1] = t1; res[2] = t2; }
+}
+
+// ---- kernel variants --------------------------------------------------------
+enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1 };
+
+struct Variant {
+  int kind;
+  int T;     // time levels fused per launch
+  int by;    // rows per lane (march kernels)
+};
+
+template <class P>
+using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
+                               int, Params, hipStream_t);
+
+template <class P, int T, int BY>
+void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int lx,
+                    int nty, int ntz, int ntx, Params prm, hipStream_t s) {
+  const int ntot = nty * ntz * ntx;
+  const int per = (ntot + N_XCD - 1) / N_XCD;
+  hipLaunchKernelGGL((march3d_kernel<P, T, BY>), dim3(per * N_XCD), dim3(WAVE), 0, s, in, out, nx, ny, nz, lx, nty,
+                     ntz, ntx, prm);
+}
+
+template <class P>
+struct VariantEntry {
+  Variant v;
+  MarchLauncher<P> fn;
+};
+
+template <class P>
+class StepLibrary {
+ public:
+  using real = typename P::real;
+  int n[3] = {1, 1, 1};
+  size_t count = 0;
+  Fields<real> buf[2];
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  Params prm{};
+  int nparams = 0;
+  const VariantEntry<P>* variants = nullptr;
+  int nvariants = 0;
+  Variant choice{KIND_DIRECT, 1, 0};
+  int lx = 64;
+  double* red_part = nullptr;
+  double* red_res = nullptr;
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ready = false;
+
+  void init(int n0, int n1, int n2, const VariantEntry<P>* vs, int nv) {
+    if (ready) return;
+    n[0] = n0; n[1] = n1; n[2] = n2;
+    count = (size_t)n0 * n1 * n2;
+    variants = vs;
+    nvariants = nv;
+    FMR_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < MAX_FIELDS; ++f) buf[b].f[f] = nullptr;
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < P::NF; ++f) {
+        FMR_CHECK(hipMalloc(&buf[b].f[f], count * sizeof(real)));
+        FMR_CHECK(hipMemsetAsync(buf[b].f[f], 0, count * sizeof(real), stream));
+      }
+    FMR_CHECK(hipMalloc(&red_part, sizeof(double) * 3 * RED_BLOCKS));
+    FMR_CHECK(hipMalloc(&red_res, sizeof(double) * 3));
+    FMR_CHECK(hipEventCreate(&ev0));
+    FMR_CHECK(hipEventCreate(&ev1));
+    // default: the fused variant with T = 2 if compiled in, else the first one
+    choice = Variant{KIND_DIRECT, 1, 0};
+    for (int i = 0; i < nvariants; ++i)
+      if (choice.kind == KIND_DIRECT || variants[i].v.T == 2) choice = variants[i].v;
+    FMR_CHECK(hipStreamSynchronize(stream));
+    ready = true;
+  }
+
+  void* host_staging(size_t bytes) {
+    if (bytes > staging_bytes) {
+      if (staging) FMR_CHECK(hipHostFree(staging));
+      FMR_CHECK(hipHostMalloc(&staging, bytes, hipHostMallocDefault));
+      staging_bytes = bytes;
+    }
+    return staging;
+  }
+
+  // host array with shape (s0, s1, s2) and interior starting at (o0, o1, o2)
+  template <class H>
+  void upload(int f, const H* host, const int s[3], const int o[3]) {
+    real* st = (real*)host_staging(count * sizeof(real));
+    for (int x = 0; x < n[0]; ++x)
+      for (int y = 0; y < n[1]; ++y) {
+        const H* src = host + ((size_t)(x + o[0]) * s[1] + (y + o[1])) * s[2] + o[2];
+        real* dst = st + ((size_t)x * n[1] + y) * n[2];
+        for (int z = 0; z < n[2]; ++z) dst[z] = (real)src[z];
+      }
+    FMR_CHECK(hipMemcpyAsync(buf[cur].f[f], st, count * sizeof(real), hipMemcpyHostToDevice, stream));
+    FMR_CHECK(hipStreamSynchronize(stream));
+  }
+
+  template <class H>
+  void download(int f, H* host, const int s[3], const int o[3]) {
+    real* st = (real*)host_staging(count * sizeof(real));
+    FMR_CHECK(hipMemcpyAsync(st, buf[cur].f[f], count * sizeof(real), hipMemcpyDeviceToHost, stream));
+    FMR_CHECK(hipStreamSynchronize(stream));
+    for (int x = 0; x < n[0]; ++x)
+      for (int y = 0; y < n[1]; ++y) {
+        H* dst = host + ((size_t)(x + o[0]) * s[1] + (y + o[1])) * s[2] + o[2];
+        const real* src = st + ((size_t)x * n[1] + y) * n[2];
+        for (int z = 0; z < n[2]; ++z) dst[z] = (H)src[z];
+      }
+  }
+
+  const VariantEntry<P>* find(int kind, int T, int by) const {
+    for (int i = 0; i < nvariants; ++i)
+      if (variants[i].v.kind == kind && variants[i].v.T == T && variants[i].v.by == by) return &variants[i];
+    return nullptr;
+  }
+
+  void launch_direct(int steps) {
+    const int threads = 256;
+    const size_t blocks_needed = (count + threads - 1) / threads;
+    const int blocks = (int)(blocks_needed < 65536 ? blocks_needed : 65536);
+    for (int s = 0; s < steps; ++s) {
+      hipLaunchKernelGGL((direct_step_kernel<P>), dim3(blocks), dim3(threads), 0, stream, buf[cur], buf[1 - cur],
+                         n[0], n[1], n[2], prm);
+      FMR_CHECK(hipGetLastError());
+      cur = 1 - cur;
+    }
+  }
+
+  void launch_march(const VariantEntry<P>* e) {
+    const int T = e->v.T, by = e->v.by;
+    const int nty = (n[1] + (by - 2 * T) - 1) / (by - 2 * T);
+    const int ntz = (n[2] + (WAVE - 2 * T) - 1) / (WAVE - 2 * T);
+    const int ntx = (n[0] + lx - 1) / lx;
+    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], lx, nty, ntz, ntx, prm, stream);
+    FMR_CHECK(hipGetLastError());
+    cur = 1 - cur;
+  }
+
+  // advance the device state by `steps` time steps (asynchronous on `stream`)
+  void advance(int steps) {
+    if (choice.kind == KIND_DIRECT) {
+      launch_direct(steps);
+      return;
+    }
+    int left = steps;
+    while (left > 0) {
+      const VariantEntry<P>* e = nullptr;
+      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by);
+      if (!e) {   // remainder: deepest variant with the same row count that fits
+        for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t) e = find(KIND_MARCH, t, choice.by);
+        for (int i = 0; i < nvariants && !e; ++i)
+          if (variants[i].v.T <= left) e = &variants[i];
+      }
+      if (!e) {
+        launch_direct(left);
+        return;
+      }
+      launch_march(e);
+      left -= e->v.T;
+    }
+  }
+
+  float timed_advance(int steps) {
+    FMR_CHECK(hipEventRecord(ev0, stream));
+    advance(steps);
+    FMR_CHECK(hipEventRecord(ev1, stream));
+    FMR_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    FMR_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    return ms;
+  }
+
+  void sync() { FMR_CHECK(hipStreamSynchronize(stream)); }
+
+  // res = {sum, sum of squares, 0}
+  void reduce(int f, double res[3]) {
+    hipLaunchKernelGGL((reduce_pass1<real, real, 0>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream,
+                       buf[cur].f[f], (const real*)nullptr, count, red_part);
+    hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(64), 0, stream, red_part, red_res);
+    FMR_CHECK(hipGetLastError());
+    FMR_CHECK(hipMemcpyAsync(res, red_res, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FMR_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // res = {sum (a-ref)^2, sum ref^2, max |a-ref|} against a device field `ref`
+  template <class R>
+  void diff(int f, const R* ref, double res[3]) {
+    hipLaunchKernelGGL((reduce_pass1<real, R, 1>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, buf[cur].f[f],
+                       ref, count, red_part);
+    hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(64), 0, stream, red_part, red_res);
+    FMR_CHECK(hipGetLastError());
+    FMR_CHECK(hipMemcpyAsync(res, red_res, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FMR_CHECK(hipStreamSynchronize(stream));
+  }
+};
+
+}  // namespace fmr

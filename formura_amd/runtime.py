@@ -1,0 +1,166 @@
+"""Python view of a step library through its C ABI (ctypes, no torch types).
+
+A step library -- the HIP one built by formura_amd.build, or any other
+library with the same generated header -- exports the state arrays, the global
+`navi`, `Formura_Init` and `Formura_Forward` (reference ABI,
+src/Formura/MPICxx/Translate.hs:1000-1139) plus the device-resident entry points
+of include/formura_amd.h.  `StepLibrary` maps the state arrays as numpy arrays
+(interior view `[lower, upper)` per axis) and calls the entry points exactly as a
+`main.cpp` would.  Loading fails loudly when the library file is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import os
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from .compiler.abi import Layout, delta_name, neighbour_deltas
+
+
+def navigator_type(axes: Sequence[str]):
+    fields = [("time_step", ctypes.c_int)]
+    for a in axes:
+        fields += [(f"lower_{a}", ctypes.c_int), (f"upper_{a}", ctypes.c_int), (f"offset_{a}", ctypes.c_int)]
+    fields += [("mpi_comm", ctypes.c_int), ("mpi_my_rank", ctypes.c_int)]
+    for d in neighbour_deltas(len(axes)):
+        fields.append((delta_name(d), ctypes.c_int))
+    return type("Formura_Navigator", (ctypes.Structure,), {"_fields_": fields})
+
+
+class StepLibrary:
+    """One loaded step library (one program, one process-global state)."""
+
+    def __init__(self, path: str, layout: Layout, device: bool = True):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"step library {path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        self.path = path
+        self.layout = layout
+        self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        self.Nav = navigator_type(layout.axes)
+        self.navi = self.Nav.in_dll(self.lib, "navi")
+        shape = layout.shape()
+        self.arrays: Dict[str, np.ndarray] = {}
+        for f, t in zip(layout.field_names, layout.field_ctype):
+            ct = ctypes.c_double if t == "double" else ctypes.c_float
+            n = int(np.prod(shape))
+            raw = (ct * n).in_dll(self.lib, f)
+            self.arrays[f] = np.ctypeslib.as_array(raw).reshape(shape)
+        self.device = device
+        L = self.lib
+        L.Formura_Init.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
+        L.Formura_Forward.argtypes = [ctypes.POINTER(self.Nav)]
+        if device:
+            L.Formura_Upload.argtypes = [ctypes.POINTER(self.Nav)]
+            L.Formura_Download.argtypes = [ctypes.POINTER(self.Nav)]
+            L.Formura_Forward_Device.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
+            L.Formura_Forward_Device_Timed.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
+            L.Formura_Forward_Device_Timed.restype = ctypes.c_float
+            L.Formura_Set_Kernel.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            L.Formura_Set_March_Chunk.argtypes = [ctypes.c_int]
+            L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 4
+            L.Formura_List_Kernels.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+            L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+            L.Formura_Device_Field.argtypes = [ctypes.c_int]
+            L.Formura_Device_Field.restype = ctypes.c_void_p
+            L.Formura_Reduce_Device.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+            L.Formura_Diff_Device.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_double)]
+
+    # -- reference ABI ------------------------------------------------------
+    def init(self) -> None:
+        rc = self.lib.Formura_Init(ctypes.byref(self.navi), 0)
+        if rc != 0:
+            raise RuntimeError(f"Formura_Init returned {rc}")
+
+    def forward(self) -> None:
+        rc = self.lib.Formura_Forward(ctypes.byref(self.navi))
+        if rc != 0:
+            raise RuntimeError(f"Formura_Forward returned {rc}")
+
+    @property
+    def time_step(self) -> int:
+        return self.navi.time_step
+
+    def interior(self, field: str) -> np.ndarray:
+        lo, hi = self.layout.lower(), self.layout.upper()
+        return self.arrays[field][tuple(slice(l, h) for l, h in zip(lo, hi))]
+
+    def set_state(self, values: Dict[str, np.ndarray]) -> None:
+        for f, v in values.items():
+            self.interior(f)[...] = v
+
+    def get_state(self) -> Dict[str, np.ndarray]:
+        return {f: self.interior(f).copy() for f in self.layout.field_names}
+
+    # -- device-resident entry points --------------------------------------
+    def upload(self) -> None:
+        self.lib.Formura_Upload(ctypes.byref(self.navi))
+
+    def download(self) -> None:
+        self.lib.Formura_Download(ctypes.byref(self.navi))
+
+    def forward_device(self, nsteps: int) -> None:
+        self.lib.Formura_Forward_Device(ctypes.byref(self.navi), int(nsteps))
+
+    def forward_device_timed(self, nsteps: int) -> float:
+        return float(self.lib.Formura_Forward_Device_Timed(ctypes.byref(self.navi), int(nsteps)))
+
+    def set_kernel(self, kind: int, T: int = 1, rows: int = 0) -> None:
+        if self.lib.Formura_Set_Kernel(kind, T, rows) != 0:
+            raise ValueError(f"kernel variant kind={kind} T={T} rows={rows} is not compiled in")
+
+    def set_chunk(self, lx: int) -> None:
+        if self.lib.Formura_Set_March_Chunk(int(lx)) != 0:
+            raise ValueError("bad march chunk")
+
+    def get_kernel(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self.lib.Formura_Get_Kernel(*[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def list_kernels(self) -> List[tuple]:
+        buf = (ctypes.c_int * 128)()
+        n = self.lib.Formura_List_Kernels(buf, 64)
+        return [(buf[2 * i], buf[2 * i + 1]) for i in range(n)]
+
+    def set_params(self, p: Sequence[float]) -> None:
+        arr = (ctypes.c_double * len(p))(*p)
+        if self.lib.Formura_Set_Params(arr, len(p)) != 0:
+            raise ValueError("too many parameters")
+
+    def device_field(self, f: int) -> int:
+        return int(self.lib.Formura_Device_Field(f))
+
+    def elem_size(self) -> int:
+        return int(self.lib.Formura_Device_Elem_Size())
+
+    def reduce(self, f: int):
+        r = (ctypes.c_double * 3)()
+        if self.lib.Formura_Reduce_Device(f, r) != 0:
+            raise ValueError("bad field")
+        return r[0], r[1]
+
+    def diff(self, f: int, ref: "StepLibrary"):
+        """(L2 of difference, L2 of ref, max abs difference) of field f against
+        the same field of another library's device state."""
+        r = (ctypes.c_double * 3)()
+        if self.lib.Formura_Diff_Device(f, ctypes.c_void_p(ref.device_field(f)), ref.elem_size(), r) != 0:
+            raise ValueError("bad diff arguments")
+        return float(np.sqrt(r[0])), float(np.sqrt(r[1])), float(r[2])
+
+
+KIND_DIRECT = 0
+KIND_MARCH = 1
+
+
+def load_program(name: str) -> StepLibrary:
+    """Load the built HIP step library of a registered program."""
+    from . import build
+    from .compiler.driver import compile_files
+    fmr, yaml_, precision, params, over, _ = build.PROGRAMS[name]
+    c = compile_files(name, fmr, yaml_, precision, params, over)
+    return StepLibrary(build.lib_path(name), c.layout, device=True)

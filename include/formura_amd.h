@@ -1,0 +1,87 @@
+/* formura_amd.h -- C ABI of a formura_amd step library (one MI355X).
+ *
+ * Every library generated from a `.fmr` program exports two groups of
+ * symbols.  Plain C types only: pointers, ints, doubles.
+ *
+ * 1. The reference ABI, declared per program in the generated `<name>.h`
+ *    (layout of src/Formura/MPICxx/Translate.hs:1000-1139):
+ *
+ *      extern double U[SX][SY][SZ]; ...   state arrays, interior [lower, upper)
+ *      extern struct Formura_Navigator navi;
+ *      int Formura_Init (struct Formura_Navigator *navi, MPI_Comm comm);
+ *          replaces the emitted Formura_Init (MPICxx/Translate.hs:1070-1092):
+ *          sets time_step = 0 and lower/upper/offset per axis, and also
+ *          allocates the device state.  Returns 0.
+ *      int Formura_Forward (struct Formura_Navigator *navi);
+ *          replaces the emitted Formura_Forward (MPICxx/Translate.hs:1106-1131):
+ *          advances the state arrays by monitorInterval2 steps (the monitor
+ *          interval rounded up to a multiple of 2T), adds that to time_step,
+ *          returns 0.  Synchronous: it uploads the interior, runs on the GPU
+ *          and downloads the interior before returning.  HIP errors abort
+ *          with a message.
+ *
+ * 2. Device-resident entry points declared below.  They have no counterpart
+ *    in the reference, whose state lives only in host arrays; they let a
+ *    driver keep the state in HBM across many steps (a reference driver
+ *    would call Formura_Forward in its time loop instead,
+ *    examples/pearson-3d-main.cpp:189-240).
+ */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct Formura_Navigator;
+
+/* Copy the interior of the host state arrays to the device state. */
+int Formura_Upload(struct Formura_Navigator *navi);
+
+/* Copy the device state into the interior of the host state arrays. */
+int Formura_Download(struct Formura_Navigator *navi);
+
+/* Advance the device state by nsteps (>= 0) steps, any count; returns after
+ * the GPU has finished.  time_step += nsteps. */
+int Formura_Forward_Device(struct Formura_Navigator *navi, int nsteps);
+
+/* As Formura_Forward_Device; returns the GPU time in milliseconds measured
+ * with HIP events on the library's stream around the launches. */
+float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
+
+/* Select the kernel: kind 0 = one site per thread (any stencil), kind 1 =
+ * temporally blocked march with T fused steps and `rows` y-rows per lane.
+ * Returns 0, or -1 if that variant is not compiled into the library. */
+int Formura_Set_Kernel(int kind, int T, int rows);
+
+/* x-extent of one march tile (planes per wavefront); returns 0 or -1. */
+int Formura_Set_March_Chunk(int lx);
+
+/* Current kernel selection. */
+int Formura_Get_Kernel(int *kind, int *T, int *rows, int *lx);
+
+/* Writes up to max_pairs (T, rows) pairs of compiled march variants into
+ * out[2*i], out[2*i+1]; returns the number written. */
+int Formura_List_Kernels(int *out, int max_pairs);
+
+/* Per-launch scalar parameters (programs compiled with named parameters);
+ * returns 0, or -1 if np exceeds 16. */
+int Formura_Set_Params(const double *p, int np);
+
+/* Device address of the current buffer of state field f (in `step` argument
+ * order), or NULL.  Valid until the next advance. */
+void *Formura_Device_Field(int f);
+
+/* Size in bytes of one device element: 8 (FP64) or 4 (FP32 bulk). */
+int Formura_Device_Elem_Size(void);
+
+/* FP64 reduction of device field f: res[0] = sum, res[1] = sum of squares
+ * (res must hold 3 doubles).  Deterministic.  Returns 0 or -1. */
+int Formura_Reduce_Device(int f, double *res);
+
+/* FP64 comparison of device field f with another device array of the same
+ * shape (element size 8 or 4): res[0] = sum (f - ref)^2, res[1] = sum ref^2,
+ * res[2] = max |f - ref|.  Returns 0 or -1. */
+int Formura_Diff_Device(int f, const void *ref, int ref_elem_size, double *res);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,60 @@
+"""ORACLE build recipe (test infrastructure only).
+
+Emits the reference-faithful C/OpenMP step library of each program with
+oracle/emit_cxx.py and compiles it with gcc -O3 -fopenmp -ffp-contract=off into
+oracle/_build/lib<name>_cpu.so.  The same header as the HIP library is used, so
+one driver or one Python harness runs either.
+
+The reference's own generated C in the reference tree
+(examples-generated/pearson.c) is NOT built: it needs its generated header
+`pearson.h` (not in the reference) and <mpi.h> (no MPI in this image), and
+writing stand-ins for either is not allowed; see the oracle notes in the design
+document.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+from formura_amd import build as product_build  # noqa: E402
+from formura_amd.compiler.driver import compile_files  # noqa: E402
+from oracle.emit_cxx import write_oracle  # noqa: E402
+
+OUT = os.path.join(HERE, "_build")
+CFLAGS = ["-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=gnu99", "-mcmodel=medium"]
+
+
+def oracle_path(name: str) -> str:
+    return os.path.join(OUT, f"lib{name}_cpu.so")
+
+
+def compiled(name: str):
+    fmr, yaml_, _precision, params, over, _ = product_build.PROGRAMS[name]
+    # the oracle always computes in the program's declared (FP64) types
+    return compile_files(name, fmr, yaml_, "faithful", params, over)
+
+
+def build_oracle(name: str) -> str:
+    c = compiled(name)
+    src = write_oracle(OUT, c.layout, c.mm)
+    out = oracle_path(name)
+    cmd = ["gcc"] + CFLAGS + ["-I", OUT, "-I", product_build.MPI_STUB, src, "-o", out + ".tmp", "-lm"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed: {' '.join(cmd)}\n{r.stdout[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(names=None):
+    return {n: build_oracle(n) for n in (names or product_build.PROGRAMS)}
+
+
+if __name__ == "__main__":
+    for k, v in build_all(sys.argv[1:] or None).items():
+        print(k, v)

@@ -1,0 +1,135 @@
+"""GPU parity: the HIP step libraries against the CPU oracle built from the
+same program, on the same seeded inputs, through the C ABI.
+
+The bar for the FP64 ("faithful") programs is bit equality (the requirement is
+1e-12 relative max-error; both sides evaluate the identical statement list
+with -ffp-contract=off, so equal bits are expected and asserted).  The FP32
+bulk libraries are compared with the FP64 oracle by relative L2 error, with
+the tolerance written in each test.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from formura_amd import build
+from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, StepLibrary, load_program
+from tests.common import gray_scott_ic, normal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+_cache = {}
+
+
+def hip(name):
+    if name not in _cache:
+        _cache[name] = load_program(name)
+        _cache[name].init()
+    return _cache[name]
+
+
+def oracle(name):
+    key = name + "/cpu"
+    if key not in _cache:
+        from oracle.build_oracle import compiled, oracle_path
+        c = compiled(name)
+        lib = StepLibrary(oracle_path(name), c.layout, device=False)
+        lib.init()
+        _cache[key] = lib
+    return _cache[key]
+
+
+def oracle_run(name, state, forwards):
+    o = oracle(name)
+    o.set_state(state)
+    t0 = o.time_step
+    for _ in range(forwards):
+        o.forward()
+    assert o.time_step - t0 == forwards * o.layout.steps_per_forward
+    return o.get_state()
+
+
+def gs_state(lib, seed):
+    u, v = gray_scott_ic(tuple(lib.layout.intra), seed)
+    return {"U": u, "V": v}
+
+
+def heat_state(lib, seed):
+    return {"u": normal(seed, tuple(lib.layout.intra))}
+
+
+def device_run(lib, state, nsteps):
+    lib.set_state(state)
+    lib.upload()
+    lib.forward_device(nsteps)
+    lib.download()
+    return lib.get_state()
+
+
+def assert_bitwise(got, want, label):
+    for f in want:
+        d = np.abs(got[f] - want[f])
+        rel = float(d.max() / max(np.abs(want[f]).max(), 1e-300))
+        assert np.array_equal(got[f], want[f]), f"{label}: field {f} differs, max rel err {rel:.3e}"
+
+
+@pytest.mark.parametrize("name,mk", [("gs3d_t", gs_state), ("heat_t48", heat_state), ("gs2d", gs_state)])
+def test_forward_matches_oracle_bitwise(name, mk):
+    lib = hip(name)
+    _say(f"[parity] Formura_Forward {name} default kernel {lib.get_kernel()}")
+    st = mk(lib, 7)
+    lib.set_state(st)
+    t0 = lib.time_step
+    lib.forward()
+    lib.forward()
+    assert lib.time_step - t0 == 2 * lib.layout.steps_per_forward
+    got = lib.get_state()
+    want = oracle_run(name, st, 2)
+    assert_bitwise(got, want, name)
+
+
+@pytest.mark.parametrize("name,mk", [("gs3d_t", gs_state), ("heat_t48", heat_state)])
+def test_every_kernel_variant_bitwise(name, mk):
+    lib = hip(name)
+    steps = lib.layout.steps_per_forward
+    st = mk(lib, 11)
+    want = oracle_run(name, st, 1)
+    variants = [(KIND_DIRECT, 1, 0)] + [(KIND_MARCH, T, by) for T, by in lib.list_kernels()]
+    for kind, T, by in variants:
+        for lx in ((64, 7) if kind == KIND_MARCH else (64,)):
+            _say(f"[parity] {name} kernel kind={kind} T={T} rows={by} lx={lx} steps={steps}")
+            lib.set_kernel(kind, T, by)
+            lib.set_chunk(lx)
+            got = device_run(lib, st, steps)
+            assert_bitwise(got, want, f"{name} kind={kind} T={T} rows={by} lx={lx}")
+    lib.set_chunk(64)
+
+
+def test_heat_fp32_bulk_error_vs_fp64_oracle():
+    lib = hip("heat_t48_f32")
+    _say("[parity] heat_t48_f32 vs FP64 oracle")
+    st = heat_state(lib, 5)
+    got = device_run(lib, st, lib.layout.steps_per_forward)
+    want = oracle_run("heat_t48", st, 1)
+    err = np.linalg.norm(got["u"] - want["u"]) / np.linalg.norm(want["u"])
+    _say(f"[parity] heat FP32 rel L2 error after {lib.layout.steps_per_forward} steps: {err:.3e}")
+    assert err < 1e-6     # FP32 rounding of N(0,1) data over 12 steps
+
+
+def test_fp64_reductions_match_numpy():
+    lib = hip("heat_t48")
+    st = heat_state(lib, 3)
+    lib.set_state(st)
+    lib.upload()
+    s, s2 = lib.reduce(0)
+    u = st["u"]
+    assert abs(s - u.sum()) <= 1e-9 * np.abs(u).sum()
+    assert abs(s2 - (u * u).sum()) <= 1e-12 * (u * u).sum()

@@ -1,0 +1,189 @@
+"""Pins for the CPU oracle (oracle/emit_cxx.py, built by oracle/build_oracle.py).
+
+Each check is chosen so that a plausible mistake -- a dropped term, a wrong
+sign or index, a swapped axis, a wrong association -- fails one of them:
+  * the reference's own generated C (examples-generated/pearson.c), evaluated
+    statement by statement, equals the oracle bit for bit (region plan, ridge
+    exchange and skewed stores included);
+  * an independent numpy restatement of the config formula, in the same
+    association order, equals the oracle bit for bit (3D Gray-Scott, heat);
+  * pure-Python loops on a tiny grid (brute force) equal the oracle bit for bit;
+  * heat: a Fourier mode decays by the closed-form factor per step;
+  * Gray-Scott: u=1, v=0 is an exact fixed point; a circular shift of the
+    input gives the circularly shifted output bit for bit;
+  * Formura_Forward advances time_step by monitorInterval2.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from formura_amd.compiler.driver import compile_source
+from formura_amd.compiler.numerical_config import NumericalConfig
+from formura_amd.runtime import StepLibrary
+from oracle.emit_cxx import write_oracle
+from tests.common import gray_scott_ic, normal
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EX = os.path.join(ROOT, "examples")
+MPI_STUB = os.path.join(ROOT, "include", "formura_serial_mpi")
+
+
+def build_oracle_variant(tmp_path, name, fmr, intra, monitor, T=1, text_override=None):
+    text = text_override or open(os.path.join(EX, fmr), encoding="utf-8").read()
+    nc = NumericalConfig(intra_node_shape=list(intra), mpi_grid_shape=[1] * len(intra),
+                         temporal_blocking_interval=T, monitor_interval=monitor)
+    c = compile_source(name, text, nc)
+    d = str(tmp_path / name)
+    src = write_oracle(d, c.layout, c.mm)
+    so = os.path.join(d, f"lib{name}.so")
+    subprocess.check_call(["gcc", "-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=gnu99",
+                           "-I", d, "-I", MPI_STUB, src, "-o", so, "-lm"])
+    lib = StepLibrary(so, c.layout, device=False)
+    lib.init()
+    return lib, c
+
+
+def lap3(q):
+    out = None
+    for ax in range(q.ndim):
+        d = (np.roll(q, -1, ax) - q) - (q - np.roll(q, 1, ax))
+        out = d if out is None else out + d
+    return out
+
+
+def gs_numpy(u, v, steps, Du, Dv, F, k, dx, dt):
+    """Gray-Scott, forward Euler, in the order the program states it:
+    dU = ((-((u*v)*v)) + (F*(1-u))) + ((Du/(dx*dx))*lap(u)); u' = u + dt*dU."""
+    cu = Du / (dx * dx)
+    cv = Dv / (dx * dx)
+    for _ in range(steps):
+        du = ((-((u * v) * v)) + (F * (1.0 - u))) + (cu * lap3(u))
+        dv = (((u * v) * v) - ((F + k) * v)) + (cv * lap3(v))
+        u, v = u + dt * du, v + dt * dv
+    return u, v
+
+
+def test_gs3d_oracle_equals_numpy_restatement(tmp_path):
+    lib, c = build_oracle_variant(tmp_path, "gsa", "gray-scott-3d-64.fmr", (12, 10, 14), 8)
+    u, v = gray_scott_ic((12, 10, 14), 3)
+    lib.set_state({"U": u, "V": v})
+    lib.forward()
+    got = lib.get_state()
+    wu, wv = gs_numpy(u, v, 8, 2e-5, 1e-5, 0.04, 0.06, 1.0 / 64.0, 1.0)
+    assert np.array_equal(got["U"], wu) and np.array_equal(got["V"], wv)
+
+
+def test_heat_oracle_equals_numpy_restatement(tmp_path):
+    lib, _ = build_oracle_variant(tmp_path, "heata", "heat-3d-512.fmr", (9, 11, 13), 6)
+    u = normal(9, (9, 11, 13))
+    lib.set_state({"u": u})
+    lib.forward()
+    w = u.copy()
+    coef = (0.1 * 1.0) / (1.0 * 1.0)
+    for _ in range(6):
+        w = w + coef * lap3(w)
+    assert np.array_equal(lib.get_state()["u"], w)
+
+
+def test_gs3d_oracle_equals_python_loops_on_tiny_grid(tmp_path):
+    n = (4, 3, 5)
+    lib, _ = build_oracle_variant(tmp_path, "gsb", "gray-scott-3d-64.fmr", n, 2)
+    u0, v0 = gray_scott_ic(n, 17, cube_frac=0.5)
+    lib.set_state({"U": u0, "V": v0})
+    lib.forward()
+    got = lib.get_state()
+    u, v = u0.tolist(), v0.tolist()
+    cu = 2e-5 / ((1.0 / 64.0) * (1.0 / 64.0))
+    cv = 1e-5 / ((1.0 / 64.0) * (1.0 / 64.0))
+
+    def lap(q, x, y, z):
+        X, Y, Z = n
+        s = (q[(x + 1) % X][y][z] - q[x][y][z]) - (q[x][y][z] - q[(x - 1) % X][y][z])
+        s = s + ((q[x][(y + 1) % Y][z] - q[x][y][z]) - (q[x][y][z] - q[x][(y - 1) % Y][z]))
+        s = s + ((q[x][y][(z + 1) % Z] - q[x][y][z]) - (q[x][y][z] - q[x][y][(z - 1) % Z]))
+        return s
+    for _ in range(2):
+        nu = [[[0.0] * n[2] for _ in range(n[1])] for _ in range(n[0])]
+        nv = [[[0.0] * n[2] for _ in range(n[1])] for _ in range(n[0])]
+        for x in range(n[0]):
+            for y in range(n[1]):
+                for z in range(n[2]):
+                    a, b = u[x][y][z], v[x][y][z]
+                    du = ((-((a * b) * b)) + (0.04 * (1.0 - a))) + (cu * lap(u, x, y, z))
+                    dv = (((a * b) * b) - ((0.04 + 0.06) * b)) + (cv * lap(v, x, y, z))
+                    nu[x][y][z] = a + 1.0 * du
+                    nv[x][y][z] = b + 1.0 * dv
+        u, v = nu, nv
+    assert np.array_equal(got["U"], np.array(u)) and np.array_equal(got["V"], np.array(v))
+
+
+def test_heat_fourier_mode_closed_form(tmp_path):
+    n = (16, 8, 12)
+    steps = 10
+    lib, _ = build_oracle_variant(tmp_path, "heatb", "heat-3d-512.fmr", n, steps)
+    k = (1, 2, 3)
+    grids = np.meshgrid(*[np.arange(s) for s in n], indexing="ij")
+    phase = sum(2 * np.pi * kk * g / s for kk, g, s in zip(k, grids, n))
+    u0 = np.cos(phase)
+    lib.set_state({"u": u0})
+    lib.forward()
+    lam = 1.0 + 0.1 * sum(2.0 * np.cos(2 * np.pi * kk / s) - 2.0 for kk, s in zip(k, n))
+    want = u0 * lam ** steps
+    assert np.abs(lib.get_state()["u"] - want).max() < 1e-13
+    # the mean is conserved up to rounding
+    assert abs(lib.get_state()["u"].sum() - u0.sum()) < 1e-10
+
+
+def test_gs_fixed_point_and_translation_covariance(tmp_path):
+    n = (10, 12, 8)
+    lib, _ = build_oracle_variant(tmp_path, "gsc", "gray-scott-3d-64.fmr", n, 4)
+    lib.set_state({"U": np.ones(n), "V": np.zeros(n)})
+    lib.forward()
+    s = lib.get_state()
+    assert np.array_equal(s["U"], np.ones(n)) and np.array_equal(s["V"], np.zeros(n))
+    u, v = gray_scott_ic(n, 23, cube_frac=0.3)
+    lib.set_state({"U": u, "V": v})
+    lib.forward()
+    a = lib.get_state()
+    shift = (3, -5, 2)
+    lib.set_state({"U": np.roll(u, shift, (0, 1, 2)), "V": np.roll(v, shift, (0, 1, 2))})
+    lib.forward()
+    b = lib.get_state()
+    assert np.array_equal(np.roll(a["U"], shift, (0, 1, 2)), b["U"])
+    assert np.array_equal(np.roll(a["V"], shift, (0, 1, 2)), b["V"])
+
+
+def test_time_step_advances_by_monitor_interval2(tmp_path):
+    lib, c = build_oracle_variant(tmp_path, "heatc", "heat-3d-512.fmr", (4, 4, 4), 5, T=2)
+    assert c.layout.steps_per_forward == 8      # 5 rounded up to a multiple of 2T = 4
+    lib.set_state({"u": np.zeros((4, 4, 4))})
+    lib.forward()
+    lib.forward()
+    assert lib.time_step == 16
+    assert (lib.navi.lower_x, lib.navi.upper_x) == (c.layout.margin[0], c.layout.margin[0] + 4)
+
+
+def test_reference_generated_c_equals_oracle_bitwise(tmp_path, reference_tree):
+    """examples-generated/pearson.c (600^2, 20 steps per Formura_Forward, its
+    own region plan and ridge exchange) against the oracle for the same .fmr."""
+    from tests.golden.pearson_c_interp import PearsonC
+    d = os.path.join(reference_tree, "examples-generated")
+    text = open(os.path.join(d, "pearson.fmr"), encoding="utf-8").read()
+    lib, c = build_oracle_variant(tmp_path, "pearson600", None, (600, 600), 20, T=1, text_override=text)
+    p = PearsonC(open(os.path.join(d, "pearson.c"), encoding="utf-8").read())
+    assert p.steps_per_forward == c.layout.steps_per_forward == 20
+    u, v = gray_scott_ic((600, 600), 42)
+    lib.set_state({"U": u, "V": v})
+    # pearson.c keeps the interior at rows/columns [2, 602) at the start of a step
+    p.arrays["U"][2:602, 2:602] = u
+    p.arrays["V"][2:602, 2:602] = v
+    for _ in range(2):
+        p.forward()
+        lib.forward()
+        assert np.array_equal(p.arrays["U"][2:602, 2:602], lib.interior("U"))
+        assert np.array_equal(p.arrays["V"][2:602, 2:602], lib.interior("V"))
+    assert lib.time_step == 40
