@@ -21,20 +21,44 @@ from typing import Dict, List, Sequence, Tuple
 from .abi import Layout, init_body
 from .pointir import PointIR, c_expr_of, imm_literal
 
-# register budget (32-bit VGPRs per lane) for the march kernel's plane windows:
-# T levels x 3 planes x rows x fields x words per element
-WINDOW_VGPR_BUDGET = 192
+# Register budget (32-bit registers per lane) of one march wavefront: the plane
+# windows (T levels x 3 planes), the prefetched plane and the row offsets.  A
+# wavefront at one wave per SIMD owns 256 VGPRs plus 256 AGPRs; the compiler
+# parks the overflow of the windows in AGPRs, and beyond this budget it would
+# spill to scratch memory.
+MARCH_REG_BUDGET = 340
+MAX_WAVES = 4        # <= 4 waves per workgroup keeps one wave per SIMD (512 registers)
 
 
-def default_march_variants(nf: int, real: str, max_t: int = 8, max_rows: int = 32):
-    """One (T, rows) variant per fusion depth T: the most rows per lane whose
-    windows fit the register budget, if that leaves at least 2 valid rows."""
+def march_regs(T: int, by: int, nf: int, words: int) -> int:
+    return (3 * T + 1) * by * nf * words + by
+
+
+def default_march_variants(nf: int, real: str, max_t: int = None):
+    """(T, rows per wave, waves) variants of the march kernel.
+
+    For each fusion depth T: a tile of up to 64 + 2T rows (64 valid rows),
+    split over the fewest waves (<= 4) whose registers fit the budget, and a
+    single-wave tile with as many rows as one wave can hold (no LDS, no
+    barrier; better for small grids)."""
     words = 2 if real == "double" else 1
+    if max_t is None:
+        max_t = 4 if real == "double" else 8
     out = []
     for T in range(1, max_t + 1):
-        rows = min(max_rows, WINDOW_VGPR_BUDGET // (3 * T * nf * words))
-        if rows >= 2 * T + 2:
-            out.append((T, rows))
+        by_max = 1
+        while march_regs(T, by_max + 1, nf, words) <= MARCH_REG_BUDGET:
+            by_max += 1
+        if by_max * MAX_WAVES < 2 * T + 2:
+            continue
+        rows = min(64 + 2 * T, by_max * MAX_WAVES)
+        W = -(-rows // by_max)
+        by = -(-rows // W)
+        if W > 1:
+            out.append((T, by, W))
+        by1 = min(by_max, 64 + 2 * T)
+        if by1 >= 2 * T + 2:
+            out.append((T, by1, 1))
     return out
 
 
@@ -111,10 +135,10 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("};")
     w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
-    for T, by in variants:
-        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}}}, &fmr::launch_march3d<Prog, {T}, {by}>}},")
+    for T, by, wv in variants:
+        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}}}, &fmr::launch_march3d<Prog, {T}, {by}, {wv}>}},")
     if not variants:
-        w("  {{fmr::KIND_DIRECT, 1, 0}, nullptr},")
+        w("  {{fmr::KIND_DIRECT, 1, 0, 1}, nullptr},")
     w("};")
     nv = len(variants)
     w(f"constexpr int kNumVariants = {nv};")
@@ -179,11 +203,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Set_Kernel (int kind, int T, int rows)")
+    w("int Formura_Set_Kernel (int kind, int T, int rows, int waves)")
     w("{")
-    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0}; return 0; }")
-    w("  if (!lib.find(kind, T, rows)) return -1;")
-    w("  lib.choice = fmr::Variant{kind, T, rows};")
+    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1}; return 0; }")
+    w("  if (!lib.find(kind, T, rows, waves)) return -1;")
+    w("  lib.choice = fmr::Variant{kind, T, rows, waves};")
     w("  return 0;")
     w("}")
     w("")
@@ -194,17 +218,18 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *lx)")
+    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *waves, int *lx)")
     w("{")
-    w("  *kind = lib.choice.kind; *T = lib.choice.T; *rows = lib.choice.by; *lx = lib.lx;")
+    w("  *kind = lib.choice.kind; *T = lib.choice.T; *rows = lib.choice.by; *waves = lib.choice.waves;")
+    w("  *lx = lib.lx;")
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_List_Kernels (int *out, int max_pairs)")
+    w("int Formura_List_Kernels (int *out, int max_variants)")
     w("{")
     w("  int k = 0;")
-    w("  for (int i = 0; i < kNumVariants && k < max_pairs; ++i, ++k) {")
-    w("    out[2 * k] = kVariants[i].v.T; out[2 * k + 1] = kVariants[i].v.by;")
+    w("  for (int i = 0; i < kNumVariants && k < max_variants; ++i, ++k) {")
+    w("    out[3 * k] = kVariants[i].v.T; out[3 * k + 1] = kVariants[i].v.by; out[3 * k + 2] = kVariants[i].v.waves;")
     w("  }")
     w("  return k;")
     w("}")

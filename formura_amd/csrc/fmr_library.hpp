@@ -76,21 +76,22 @@ enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1 };
 
 struct Variant {
   int kind;
-  int T;     // time levels fused per launch
-  int by;    // rows per lane (march kernels)
+  int T;       // time levels fused per launch
+  int by;      // rows per lane (march kernels)
+  int waves;   // wavefronts per tile (march kernels)
 };
 
 template <class P>
 using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
                                int, Params, hipStream_t);
 
-template <class P, int T, int BY>
+template <class P, int T, int BY, int W>
 void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int lx,
                     int nty, int ntz, int ntx, Params prm, hipStream_t s) {
   const int ntot = nty * ntz * ntx;
   const int per = (ntot + N_XCD - 1) / N_XCD;
-  hipLaunchKernelGGL((march3d_kernel<P, T, BY>), dim3(per * N_XCD), dim3(WAVE), 0, s, in, out, nx, ny, nz, lx, nty,
-                     ntz, ntx, prm);
+  hipLaunchKernelGGL((march3d_kernel<P, T, BY, W>), dim3(per * N_XCD), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
+                     lx, nty, ntz, ntx, prm);
 }
 
 template <class P>
@@ -112,7 +113,7 @@ class StepLibrary {
   int nparams = 0;
   const VariantEntry<P>* variants = nullptr;
   int nvariants = 0;
-  Variant choice{KIND_DIRECT, 1, 0};
+  Variant choice{KIND_DIRECT, 1, 0, 1};
   int lx = 64;
   double* red_part = nullptr;
   double* red_res = nullptr;
@@ -140,9 +141,9 @@ class StepLibrary {
     FMR_CHECK(hipEventCreate(&ev0));
     FMR_CHECK(hipEventCreate(&ev1));
     // default: the fused variant with T = 2 if compiled in, else the first one
-    choice = Variant{KIND_DIRECT, 1, 0};
+    choice = Variant{KIND_DIRECT, 1, 0, 1};
     for (int i = 0; i < nvariants; ++i)
-      if (choice.kind == KIND_DIRECT || variants[i].v.T == 2) choice = variants[i].v;
+      if (choice.kind == KIND_DIRECT || (variants[i].v.T == 2 && choice.T != 2)) choice = variants[i].v;
     FMR_CHECK(hipStreamSynchronize(stream));
     ready = true;
   }
@@ -183,9 +184,11 @@ class StepLibrary {
       }
   }
 
-  const VariantEntry<P>* find(int kind, int T, int by) const {
+  const VariantEntry<P>* find(int kind, int T, int by, int waves) const {
     for (int i = 0; i < nvariants; ++i)
-      if (variants[i].v.kind == kind && variants[i].v.T == T && variants[i].v.by == by) return &variants[i];
+      if (variants[i].v.kind == kind && variants[i].v.T == T && variants[i].v.by == by &&
+          variants[i].v.waves == waves)
+        return &variants[i];
     return nullptr;
   }
 
@@ -202,8 +205,8 @@ class StepLibrary {
   }
 
   void launch_march(const VariantEntry<P>* e) {
-    const int T = e->v.T, by = e->v.by;
-    const int nty = (n[1] + (by - 2 * T) - 1) / (by - 2 * T);
+    const int T = e->v.T, rows = e->v.by * e->v.waves;
+    const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
     const int ntz = (n[2] + (WAVE - 2 * T) - 1) / (WAVE - 2 * T);
     const int ntx = (n[0] + lx - 1) / lx;
     e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], lx, nty, ntz, ntx, prm, stream);
@@ -220,9 +223,10 @@ class StepLibrary {
     int left = steps;
     while (left > 0) {
       const VariantEntry<P>* e = nullptr;
-      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by);
-      if (!e) {   // remainder: deepest variant with the same row count that fits
-        for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t) e = find(KIND_MARCH, t, choice.by);
+      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by, choice.waves);
+      if (!e) {   // remainder: deepest variant with the same tile shape that fits
+        for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t)
+          e = find(KIND_MARCH, t, choice.by, choice.waves);
         for (int i = 0; i < nvariants && !e; ++i)
           if (variants[i].v.T <= left) e = &variants[i];
       }

@@ -125,40 +125,64 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // ===========================================================================
 // Kernel 2: 3D temporally blocked march (star stencils of radius 1).
 //
-// One 64-lane wavefront owns a tile of 64 z-columns (one per lane, z is the
-// contiguous axis) by BY y-rows (all rows in the lane's registers) and marches
-// along x over a chunk of LX planes.  T time levels are pipelined along the
-// march: level l computes plane (xin - l) as soon as level l-1 has produced
-// plane (xin - l + 1).  Per level a lane keeps a 3-plane window (x-1, x, x+1)
-// of its BY rows in registers.  Neighbours:
+// A workgroup of W wavefronts owns a tile of 64 z-columns (one per lane; z is
+// the contiguous axis) by W*BY y-rows (wave w holds rows [w*BY, (w+1)*BY) in
+// its registers) and marches along x over a chunk of planes.  T time levels
+// are pipelined along the march: level l computes plane (xin - l) as soon as
+// level l-1 has produced plane (xin - l + 1).  Per level a lane keeps a
+// 3-plane window (x-1, x, x+1) of its BY rows in registers.  Neighbours:
 //   x +-1 : the window (registers),
-//   y +-1 : the neighbouring row of the same lane (registers),
-//   z +-1 : the neighbouring lane (DPP wave_shr:1 / wave_shl:1),
-// so the kernel uses no LDS and no barriers.  Ghost zones: the tile carries
-// T halo rows / lanes on each side, recomputed redundantly; after level l the
-// valid part is rows [l, BY-l) and lanes [l, 64-l).  Only level T's valid part
-// is written.  Reads wrap periodically, so the ghost zones also implement the
-// periodic boundary.  The window slot of the newest plane rotates with the
-// iteration phase (0,1,2), so the march loop is unrolled by 3 and no register
-// moves are needed.
+//   y +-1 : the neighbouring row of the same lane (registers); across a wave
+//           boundary, the neighbouring wave's edge row, exchanged through LDS,
+//   z +-1 : the neighbouring lane (DPP wave_shr:1 / wave_shl:1).
+// The edge rows every level needs in one iteration are all known when the
+// iteration starts (they are the window centres, produced one iteration
+// earlier), so one LDS exchange and one barrier per plane serve all T levels;
+// the LDS buffer alternates between two halves so the next plane's writes
+// cannot overtake this plane's reads.  With W == 1 the kernel uses no LDS
+// and no barrier.
+// Ghost zones: the tile carries T ghost rows / lanes on each side, recomputed
+// redundantly; after level l the valid part is rows [l, W*BY-l) and lanes
+// [l, 64-l).  Only level T's valid part is written.  Reads wrap periodically,
+// so the ghost zones also implement the periodic boundary.  The window slot
+// of the newest plane rotates with the iteration phase (0,1,2), so the march
+// loop is unrolled by 3 and no register moves are needed.
 //
 // Workgroup -> tile mapping is XCD-aware: consecutive workgroups go to
 // different XCDs, so tile t is given to workgroup (t % per)*8 + t / per, which
 // keeps neighbouring tiles -- whose ghost rows overlap -- on one XCD and its L2.
 // ===========================================================================
-template <class P, int T, int BY>
+template <class P, int T, int BY, int W>
 struct March3D {
   using real = typename P::real;
   static constexpr int NF = P::NF;
   static constexpr int H = T;
-  static_assert(BY > 2 * T, "tile must be taller than its ghost rows");
+  static constexpr int ROWS = BY * W;
+  static_assert(ROWS > 2 * T, "tile must be taller than its ghost rows");
+  using Edge = real[2][T][W][2][NF][WAVE];   // [half][level][wave][top,bottom][field][lane]
 
   real w[T][3][BY][NF];
   real nxt[BY][NF];
+  Edge* edge;   // LDS (W > 1 only)
+  int wave, lane, half;
+  int ylim;     // tile rows >= ylim lie past the grid end (periodic duplicates): never stored
+
+  template <int L, int CEN, int ROW, int F>
+  __device__ __forceinline__ real row_above() const {
+    if constexpr (ROW > 0) return w[L][CEN][ROW - 1][F];
+    else if constexpr (W == 1) return w[L][CEN][ROW][F];   // a ghost row: any finite value
+    else return wave > 0 ? (*edge)[half][L][wave - 1][1][F][lane] : w[L][CEN][ROW][F];
+  }
+  template <int L, int CEN, int ROW, int F>
+  __device__ __forceinline__ real row_below() const {
+    if constexpr (ROW < BY - 1) return w[L][CEN][ROW + 1][F];
+    else if constexpr (W == 1) return w[L][CEN][ROW][F];
+    else return wave < W - 1 ? (*edge)[half][L][wave + 1][0][F][lane] : w[L][CEN][ROW][F];
+  }
 
   template <int L, int PH, int ROW>
   struct Acc {
-    March3D& m;
+    const March3D& m;
     const Params& prm;
     int cx, cy, cz;
     real out[NF];
@@ -169,8 +193,8 @@ struct March3D {
       constexpr int NEW = PH, CEN = (PH + 2) % 3, OLD = (PH + 1) % 3;
       if constexpr (D0 == -1) return m.w[L][OLD][ROW][F];
       else if constexpr (D0 == 1) return m.w[L][NEW][ROW][F];
-      else if constexpr (D1 == -1) return m.w[L][CEN][ROW - 1][F];
-      else if constexpr (D1 == 1) return m.w[L][CEN][ROW + 1][F];
+      else if constexpr (D1 == -1) return m.template row_above<L, CEN, ROW, F>();
+      else if constexpr (D1 == 1) return m.template row_below<L, CEN, ROW, F>();
       else if constexpr (D2 == -1) return lane_prev(m.w[L][CEN][ROW][F]);
       else if constexpr (D2 == 1) return lane_next(m.w[L][CEN][ROW][F]);
       else return m.w[L][CEN][ROW][F];
@@ -181,27 +205,31 @@ struct March3D {
     __device__ __forceinline__ void st(real v) { out[F] = v; }
   };
 
-  __device__ __forceinline__ void load_plane(const Fields<real>& in, size_t xoff, const size_t (&yoff)[BY]) {
+  __device__ __forceinline__ void load_plane(const Fields<real>& in, size_t xoff, const int (&yoff)[BY]) {
 #pragma unroll
     for (int r = 0; r < BY; ++r)
 #pragma unroll
       for (int f = 0; f < NF; ++f) nxt[r][f] = in.f[f][xoff + yoff[r]];
   }
 
-  // level L+1 at rows [L+1, BY-L-1) computed from window L
-  int ylim;   // rows >= ylim lie past the grid end (periodic duplicates): never stored
+  // rows of level L+1 computed from window L: with one wave the valid rows
+  // [L+1, BY-L-1) are known at compile time; with several, every row is
+  // computed and the stores filter.
+  static constexpr int row_lo(int L) { return W == 1 ? L + 1 : 0; }
+  static constexpr int row_hi(int L) { return W == 1 ? BY - L - 1 : BY; }
 
   template <int L, int PH, int ROW>
   __device__ __forceinline__ void level_row(const Params& prm, int q, int y0, int zg, Fields<real>& out,
-                                            size_t qoff, const size_t (&yoff)[BY], bool st_ok) {
-    if constexpr (ROW < BY - L - 1) {
-      Acc<L, PH, ROW> a{*this, prm, q, y0 + ROW, zg, {}};
+                                            size_t qoff, const int (&yoff)[BY], bool st_ok) {
+    if constexpr (ROW < row_hi(L)) {
+      const int trow = wave * BY + ROW;   // row within the tile
+      Acc<L, PH, ROW> a{*this, prm, q, y0 + trow, zg, {}};
       P::point(a);
       if constexpr (L + 1 < T) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) w[L + 1][PH][ROW][f] = a.out[f];
       } else {
-        if (st_ok && ROW >= T && ROW < BY - T && ROW < ylim) {
+        if (st_ok && trow >= T && trow < ROWS - T && trow < ylim) {
 #pragma unroll
           for (int f = 0; f < NF; ++f) out.f[f][qoff + yoff[ROW]] = a.out[f];
         }
@@ -211,9 +239,8 @@ struct March3D {
   }
 
   template <int L, int PH>
-  __device__ __forceinline__ void levels(const Params& prm, int xin, int x0, int xend, int nx, int y0, int zg,
-                                         bool lane_ok, Fields<real>& out, size_t plane,
-                                         const size_t (&yoff)[BY]) {
+  __device__ __forceinline__ void levels(const Params& prm, int xin, int x0, int xend, int y0, int zg,
+                                         bool lane_ok, Fields<real>& out, size_t plane, const int (&yoff)[BY]) {
     if constexpr (L < T) {
       const int q = xin - (L + 1);   // plane computed by level L+1
       bool st_ok = false;
@@ -222,50 +249,71 @@ struct March3D {
         st_ok = lane_ok && q >= x0 && q < xend;
         qoff = (size_t)(st_ok ? q : 0) * plane;
       }
-      level_row<L, PH, L + 1>(prm, q, y0, zg, out, qoff, yoff, st_ok);
-      levels<L + 1, PH>(prm, xin, x0, xend, nx, y0, zg, lane_ok, out, plane, yoff);
+      level_row<L, PH, row_lo(L)>(prm, q, y0, zg, out, qoff, yoff, st_ok);
+      levels<L + 1, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane, yoff);
     }
   }
 
   template <int PH>
   __device__ __forceinline__ void iter(const Fields<real>& in, Fields<real>& out, const Params& prm, int xin,
                                        int x0, int xend, int nx, int y0, int zg, bool lane_ok, size_t plane,
-                                       const size_t (&yoff)[BY]) {
+                                       const int (&yoff)[BY]) {
+    constexpr int CEN = (PH + 2) % 3;
 #pragma unroll
     for (int r = 0; r < BY; ++r)
 #pragma unroll
       for (int f = 0; f < NF; ++f) w[0][PH][r][f] = nxt[r][f];
+    if constexpr (W > 1) {
+      // publish this wave's edge rows of every window centre, then one barrier
+#pragma unroll
+      for (int l = 0; l < T; ++l)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          (*edge)[half][l][wave][0][f][lane] = w[l][CEN][0][f];
+          (*edge)[half][l][wave][1][f][lane] = w[l][CEN][BY - 1][f];
+        }
+      __syncthreads();
+    }
     load_plane(in, (size_t)wrap(xin + 1, nx) * plane, yoff);
-    levels<0, PH>(prm, xin, x0, xend, nx, y0, zg, lane_ok, out, plane, yoff);
+    levels<0, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane, yoff);
+    half ^= 1;
   }
 };
 
-template <class P, int T, int BY>
-__global__ void __launch_bounds__(64) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
-                                                     int nx, int ny, int nz, int lx, int nty, int ntz, int ntx,
-                                                     Params prm) {
-  using M = March3D<P, T, BY>;
+template <class P, int T, int BY, int W>
+__global__ void __launch_bounds__(64 * W) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
+                                                         int nx, int ny, int nz, int lx, int nty, int ntz, int ntx,
+                                                         Params prm) {
+  using M = March3D<P, T, BY, W>;
   constexpr int H = T;
   const int ntot = nty * ntz * ntx;
   const int per = (ntot + N_XCD - 1) / N_XCD;
   const int b = blockIdx.x;
   const int tile = (b % N_XCD) * per + b / N_XCD;
-  if (tile >= ntot) return;
+  if (tile >= ntot) return;   // uniform over the workgroup: no barrier is skipped by part of it
   const int tz = tile % ntz;
   const int ty = (tile / ntz) % nty;
   const int tx = tile / (ntz * nty);
-  const int lane = threadIdx.x;
-  const int y0 = ty * (BY - 2 * H) - H;
-  const int zu = tz * (WAVE - 2 * H) - H + lane;   // unwrapped z of this lane
+  M m;
+  m.wave = threadIdx.x / WAVE;
+  m.lane = threadIdx.x % WAVE;
+  m.half = 0;
+  if constexpr (W > 1) {
+    __shared__ typename M::Edge edge_buf;
+    m.edge = &edge_buf;
+  } else {
+    m.edge = nullptr;
+  }
+  const int y0 = ty * (M::ROWS - 2 * H) - H;          // tile row 0
+  const int zu = tz * (WAVE - 2 * H) - H + m.lane;     // unwrapped z of this lane
   const int zg = wrap(zu, nz);
   const int x0 = tx * lx;
   const int xend = min(x0 + lx, nx);
-  const bool lane_ok = lane >= H && lane < WAVE - H && zu < nz;
+  const bool lane_ok = m.lane >= H && m.lane < WAVE - H && zu < nz;
   const size_t plane = (size_t)ny * nz;
-  size_t yoff[BY];
+  int yoff[BY];   // offset of row r within a plane (a plane holds < 2^31 sites)
 #pragma unroll
-  for (int r = 0; r < BY; ++r) yoff[r] = (size_t)wrap(y0 + r, ny) * nz + zg;
-  M m;
+  for (int r = 0; r < BY; ++r) yoff[r] = wrap(y0 + m.wave * BY + r, ny) * nz + zg;
   m.ylim = ny - y0;
   m.load_plane(in, (size_t)wrap(x0 - H, nx) * plane, yoff);
   const int iters = xend - x0 + 2 * T;
