@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the generated explicit step on one MI355X.
+
+Headline workload (one line of JSON on stdout): the 3D heat equation
+u_t = D lap(u), 7-point stencil, 512^3 periodic grid, FP64, D=1, dx=1, dt=0.1,
+initial u i.i.d. N(0,1) -- compiled from examples/heat-3d-512.fmr into a HIP
+step library.  A "step" is one time step over the whole grid; `value` is
+lattice-site updates per second (GLUP/s) with the state resident in HBM.
+The kernel variant (fusion depth T, tile shape, march chunk) is chosen by a
+short timed search before the timed region; the best rate for each fusion depth
+T in {1, 2, 4, 8} is reported as well.
+
+Also reported (same line, key "extra"): the FP32 bulk heat rate, 3D Gray-Scott
+512^3 in FP64 and FP32, and 3D Gray-Scott 64^3 FP64 (1000 steps).
+
+cpu_baseline: the reference-faithful C/OpenMP oracle library of the same
+program, built with gcc -O3 -fopenmp -ffp-contract=off, on all host cores, on
+a bounded sample (20 steps of the full 512^3 grid).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, published
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def seeded_state(lib, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shape = tuple(lib.layout.intra)
+    if lib.layout.field_names == ["u"]:
+        return {"u": rng.standard_normal(shape)}
+    from tests.common import gray_scott_ic
+    u, v = gray_scott_ic(shape, seed)
+    return {"U": u, "V": v}
+
+
+def candidates(lib):
+    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH
+    out = [((KIND_DIRECT, 1, 0, 1, 1), 64)]
+    for v in lib.list_kernels():
+        for lx in (32, 64, 128):
+            out.append(((KIND_MARCH,) + tuple(v), lx))
+    return out
+
+
+def autotune(lib, probe_steps=12):
+    """Time every compiled variant briefly; return {T: (GLUP/s, variant, lx)}."""
+    import numpy as np
+    cells = int(np.prod(lib.layout.intra))
+    best = {}
+    for var, lx in candidates(lib):
+        T = var[1]
+        lib.set_kernel(*var)
+        lib.set_chunk(lx)
+        steps = max(T, (probe_steps // T) * T)
+        lib.forward_device_timed(T)
+        ms = lib.forward_device_timed(steps)
+        rate = steps * cells / (ms * 1e-3) / 1e9
+        if T not in best or rate > best[T][0]:
+            best[T] = (rate, var, lx)
+    return best
+
+
+def timed_run(lib, var, lx, steps, warmup):
+    """Warm up, then time exactly `steps` steps (device synchronised on both
+    sides).  Returns (wall seconds, GPU event milliseconds)."""
+    lib.set_kernel(*var)
+    lib.set_chunk(lx)
+    if warmup:
+        lib.forward_device(warmup)          # returns after the GPU is idle
+    t0 = time.perf_counter()
+    ev_ms = lib.forward_device_timed(steps)  # synchronises on the library stream
+    t1 = time.perf_counter()
+    return t1 - t0, ev_ms
+
+
+def cpu_baseline(sample_steps=20):
+    """Oracle C/OpenMP library on the host cores, 512^3 heat, `sample_steps`."""
+    import numpy as np
+    from oracle.build_oracle import ORACLE_ONLY, compiled_extra, oracle_path
+    from formura_amd.runtime import StepLibrary
+    name = "heat512_cpu"
+    c = compiled_extra(name)
+    path = oracle_path(name)
+    lib = StepLibrary(path, c.layout, device=False)
+    lib.init()
+    rng = np.random.default_rng(1)
+    lib.set_state({"u": rng.standard_normal(tuple(c.layout.intra))})
+    per = c.layout.steps_per_forward
+    lib.forward()                       # warm-up (first touch, thread start)
+    n = max(1, sample_steps // per)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        lib.forward()
+    dt = time.perf_counter() - t0
+    cells = int(np.prod(c.layout.intra))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(n * per * cells / dt / 1e9, 4), "unit": "GLUP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n * per} steps of heat 512^3 FP64 (oracle C/OpenMP, gcc -O3 -fopenmp "
+                      f"-ffp-contract=off), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,lx: skip the search")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
+              "not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from formura_amd.runtime import load_program
+
+    lib = load_program("heat512")
+    lib.init()
+    cells = int(np.prod(lib.layout.intra))
+    lib.set_state(seeded_state(lib, 1))
+    lib.upload()
+    sweep = {}
+    if a.variant:
+        v = [int(x) for x in a.variant.split(",")]
+        var, lx = tuple(v[:5]), v[5]
+    else:
+        log("autotuning heat512 FP64 ...")
+        best = autotune(lib)
+        sweep = {T: round(r, 1) for T, (r, _, _) in sorted(best.items())}
+        _, var, lx = max(best.values())
+        log(f"best per T: {sweep}; chosen {var} lx={lx}")
+    T = var[1]
+    wall, ev_ms = timed_run(lib, var, lx, a.steps, a.warmup)
+    glups = a.steps * cells / wall / 1e9
+    launches = -(-a.steps // T)
+    bytes_per_launch = 2 * 8 * cells          # read + write of the FP64 state once per fused launch
+    launch_ms = ev_ms / launches
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    s0, _ = lib.reduce(0)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_heat512.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            t = json.load(f)
+        if t.get("variant") == list(var) + [lx]:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "GLUP/s & %HBM roofline; batched/surrogate speedup at <=1e-3 rel L2 vs FP64",
+        "value": round(glups, 2),
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: u ~ N(0,1) i.i.d. (numpy PCG64 seed 1)",
+        "config": {"workload": "heat3d 7-point, 512^3 periodic, FP64, D=1 dx=1 dt=0.1",
+                   "grid": [512, 512, 512], "kernel": {"kind": var[0], "T": var[1], "rows_per_wave": var[2],
+                                                       "waves": var[3], "prefetch": var[4], "march_chunk": lx}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "t_sweep_glups": sweep,
+        "kernel_ms_per_launch": round(launch_ms, 5),
+        "sum_u_after": s0,
+    }
+
+    if not a.no_extra:
+        extra = {}
+        for name, label in (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp64"),
+                            ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64")):
+            lg = load_program(name)
+            lg.init()
+            lg.set_state(seeded_state(lg, 1))
+            lg.upload()
+            best = autotune(lg, probe_steps=12 if "512" in name else 100)
+            r, v, l = max(best.values())
+            steps = 100 if "512" in name else 1000
+            w, e = timed_run(lg, v, l, steps, 10)
+            n = int(np.prod(lg.layout.intra))
+            extra[label] = {"glups": round(steps * n / w / 1e9, 2), "kernel": list(v) + [l],
+                            "best_per_T": {T: round(x[0], 1) for T, x in sorted(best.items())}}
+            log(f"{label}: {extra[label]}")
+        out["extra"] = extra
+
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

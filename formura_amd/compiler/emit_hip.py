@@ -30,17 +30,17 @@ MARCH_REG_BUDGET = 340
 MAX_WAVES = 4        # <= 4 waves per workgroup keeps one wave per SIMD (512 registers)
 
 
-def march_regs(T: int, by: int, nf: int, words: int) -> int:
-    return (3 * T + 1) * by * nf * words + by
+def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
+    return (3 * T + pf) * by * nf * words + by
 
 
 def default_march_variants(nf: int, real: str, max_t: int = None):
-    """(T, rows per wave, waves) variants of the march kernel.
+    """(T, rows per wave, waves, prefetch depth) variants of the march kernel.
 
     For each fusion depth T: a tile of up to 64 + 2T rows (64 valid rows),
-    split over the fewest waves (<= 4) whose registers fit the budget, and a
-    single-wave tile with as many rows as one wave can hold (no LDS, no
-    barrier; better for small grids)."""
+    split over the fewest waves (<= 4) whose registers fit the budget; and
+    single-wave tiles (no LDS, no barrier) with as many rows as one wave can
+    hold, prefetching one or two planes ahead."""
     words = 2 if real == "double" else 1
     if max_t is None:
         max_t = 4 if real == "double" else 8
@@ -55,10 +55,15 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
         W = -(-rows // by_max)
         by = -(-rows // W)
         if W > 1:
-            out.append((T, by, W))
+            out.append((T, by, W, 1))
         by1 = min(by_max, 64 + 2 * T)
         if by1 >= 2 * T + 2:
-            out.append((T, by1, 1))
+            out.append((T, by1, 1, 1))
+        by2 = 1
+        while march_regs(T, by2 + 1, nf, words, 2) <= MARCH_REG_BUDGET:
+            by2 += 1
+        if by2 >= 2 * T + 2:
+            out.append((T, min(by2, 64 + 2 * T), 1, 2))
     return out
 
 
@@ -135,10 +140,10 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("};")
     w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
-    for T, by, wv in variants:
-        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}}}, &fmr::launch_march3d<Prog, {T}, {by}, {wv}>}},")
+    for T, by, wv, pf in variants:
+        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}}}, &fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}>}},")
     if not variants:
-        w("  {{fmr::KIND_DIRECT, 1, 0, 1}, nullptr},")
+        w("  {{fmr::KIND_DIRECT, 1, 0, 1, 1}, nullptr},")
     w("};")
     nv = len(variants)
     w(f"constexpr int kNumVariants = {nv};")
@@ -203,11 +208,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Set_Kernel (int kind, int T, int rows, int waves)")
+    w("int Formura_Set_Kernel (int kind, int T, int rows, int waves, int pf)")
     w("{")
-    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1}; return 0; }")
-    w("  if (!lib.find(kind, T, rows, waves)) return -1;")
-    w("  lib.choice = fmr::Variant{kind, T, rows, waves};")
+    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1}; return 0; }")
+    w("  if (!lib.find(kind, T, rows, waves, pf)) return -1;")
+    w("  lib.choice = fmr::Variant{kind, T, rows, waves, pf};")
     w("  return 0;")
     w("}")
     w("")
@@ -218,9 +223,10 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *waves, int *lx)")
+    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *waves, int *pf, int *lx)")
     w("{")
     w("  *kind = lib.choice.kind; *T = lib.choice.T; *rows = lib.choice.by; *waves = lib.choice.waves;")
+    w("  *pf = lib.choice.pf;")
     w("  *lx = lib.lx;")
     w("  return 0;")
     w("}")
@@ -229,7 +235,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("{")
     w("  int k = 0;")
     w("  for (int i = 0; i < kNumVariants && k < max_variants; ++i, ++k) {")
-    w("    out[3 * k] = kVariants[i].v.T; out[3 * k + 1] = kVariants[i].v.by; out[3 * k + 2] = kVariants[i].v.waves;")
+    w("    out[4 * k] = kVariants[i].v.T; out[4 * k + 1] = kVariants[i].v.by; out[4 * k + 2] = kVariants[i].v.waves;")
+    w("    out[4 * k + 3] = kVariants[i].v.pf;")
     w("  }")
     w("  return k;")
     w("}")

@@ -148,11 +148,16 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // of the newest plane rotates with the iteration phase (0,1,2), so the march
 // loop is unrolled by 3 and no register moves are needed.
 //
+// The input planes are prefetched PF (1 or 2) iterations ahead, so that a
+// wavefront alone on its SIMD has one or two planes of compute to hide each
+// load; results are written with non-temporal stores so they do not evict
+// the ghost rows the neighbouring tiles are about to read from L2.
+//
 // Workgroup -> tile mapping is XCD-aware: consecutive workgroups go to
 // different XCDs, so tile t is given to workgroup (t % per)*8 + t / per, which
 // keeps neighbouring tiles -- whose ghost rows overlap -- on one XCD and its L2.
 // ===========================================================================
-template <class P, int T, int BY, int W>
+template <class P, int T, int BY, int W, int PF>
 struct March3D {
   using real = typename P::real;
   static constexpr int NF = P::NF;
@@ -162,7 +167,7 @@ struct March3D {
   using Edge = real[2][T][W][2][NF][WAVE];   // [half][level][wave][top,bottom][field][lane]
 
   real w[T][3][BY][NF];
-  real nxt[BY][NF];
+  real nxt[PF][BY][NF];   // planes prefetched PF iterations ahead
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
   int ylim;     // tile rows >= ylim lie past the grid end (periodic duplicates): never stored
@@ -205,11 +210,12 @@ struct March3D {
     __device__ __forceinline__ void st(real v) { out[F] = v; }
   };
 
+  template <int K>
   __device__ __forceinline__ void load_plane(const Fields<real>& in, size_t xoff, const int (&yoff)[BY]) {
 #pragma unroll
     for (int r = 0; r < BY; ++r)
 #pragma unroll
-      for (int f = 0; f < NF; ++f) nxt[r][f] = in.f[f][xoff + yoff[r]];
+      for (int f = 0; f < NF; ++f) nxt[K][r][f] = in.f[f][xoff + yoff[r]];
   }
 
   // rows of level L+1 computed from window L: with one wave the valid rows
@@ -231,7 +237,7 @@ struct March3D {
       } else {
         if (st_ok && trow >= T && trow < ROWS - T && trow < ylim) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) out.f[f][qoff + yoff[ROW]] = a.out[f];
+          for (int f = 0; f < NF; ++f) __builtin_nontemporal_store(a.out[f], &out.f[f][qoff + yoff[ROW]]);
         }
       }
       level_row<L, PH, ROW + 1>(prm, q, y0, zg, out, qoff, yoff, st_ok);
@@ -262,7 +268,11 @@ struct March3D {
 #pragma unroll
     for (int r = 0; r < BY; ++r)
 #pragma unroll
-      for (int f = 0; f < NF; ++f) w[0][PH][r][f] = nxt[r][f];
+      for (int f = 0; f < NF; ++f) {
+        w[0][PH][r][f] = nxt[0][r][f];
+#pragma unroll
+        for (int k = 0; k + 1 < PF; ++k) nxt[k][r][f] = nxt[k + 1][r][f];
+      }
     if constexpr (W > 1) {
       // publish this wave's edge rows of every window centre, then one barrier
 #pragma unroll
@@ -274,17 +284,17 @@ struct March3D {
         }
       __syncthreads();
     }
-    load_plane(in, (size_t)wrap(xin + 1, nx) * plane, yoff);
+    load_plane<PF - 1>(in, (size_t)wrap(xin + PF, nx) * plane, yoff);
     levels<0, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane, yoff);
     half ^= 1;
   }
 };
 
-template <class P, int T, int BY, int W>
+template <class P, int T, int BY, int W, int PF>
 __global__ void __launch_bounds__(64 * W) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
                                                          int nx, int ny, int nz, int lx, int nty, int ntz, int ntx,
                                                          Params prm) {
-  using M = March3D<P, T, BY, W>;
+  using M = March3D<P, T, BY, W, PF>;
   constexpr int H = T;
   const int ntot = nty * ntz * ntx;
   const int per = (ntot + N_XCD - 1) / N_XCD;
@@ -315,7 +325,9 @@ __global__ void __launch_bounds__(64 * W) march3d_kernel(Fields<typename P::real
 #pragma unroll
   for (int r = 0; r < BY; ++r) yoff[r] = wrap(y0 + m.wave * BY + r, ny) * nz + zg;
   m.ylim = ny - y0;
-  m.load_plane(in, (size_t)wrap(x0 - H, nx) * plane, yoff);
+  m.template load_plane<0>(in, (size_t)wrap(x0 - H, nx) * plane, yoff);
+  if constexpr (PF > 1) m.template load_plane<PF - 1>(in, (size_t)wrap(x0 - H + 1, nx) * plane, yoff);
+  static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
   const int iters = xend - x0 + 2 * T;
   for (int s = 0; s < iters; s += 3) {
     const int xin = x0 - H + s;
