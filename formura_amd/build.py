@@ -87,8 +87,19 @@ def build_program(name: str) -> str:
     return compile_hip(src, lib_path(name))
 
 
+def build_copy_probe() -> str:
+    """The HBM copy calibration binary (formura_amd/_build/hbm_copy)."""
+    out = os.path.join(BUILD, "hbm_copy")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", os.path.join(CSRC, "hbm_copy.hip"), "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(names=None, jobs: int = 8) -> Dict[str, str]:
     os.makedirs(BUILD, exist_ok=True)
+    if names is None:
+        build_copy_probe()
     names = list(names or PROGRAMS)
     for n in names:      # code generation is quick and serial
         generate(n)

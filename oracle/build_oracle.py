@@ -29,10 +29,16 @@ OUT = os.path.join(HERE, "_build")
 CFLAGS = ["-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=gnu99", "-mcmodel=medium"]
 
 
-# oracle-only programs: (registered program, numerical-config overrides)
+# oracle-only programs: (registered program, numerical-config overrides,
+# parameter names)
+_GS2D_PARAMS = ("F", "k", "dt", "cu", "cv")
 ORACLE_ONLY = {
     # the 512^3 heat program with 10 steps per Formura_Forward: the CPU-baseline sample
-    "heat512_cpu": ("heat512", {"monitor_interval": 10}),
+    "heat512_cpu": ("heat512", {"monitor_interval": 10}, ()),
+    # 2D Gray-Scott with per-run parameters, for the batched-launch parity tests
+    "gs2d_p256": ("gs2d", {"monitor_interval": 40}, _GS2D_PARAMS),
+    "gs2d_p96x160": ("gs2d", {"intra_node_shape": [96, 160], "monitor_interval": 40}, _GS2D_PARAMS),
+    "gs2d_p64x32": ("gs2d", {"intra_node_shape": [64, 32], "monitor_interval": 40}, _GS2D_PARAMS),
 }
 
 
@@ -47,16 +53,17 @@ def compiled(name: str):
 
 
 def compiled_extra(name: str):
-    base, over = ORACLE_ONLY[name]
-    fmr, yaml_, _precision, params, over0, _ = product_build.PROGRAMS[base]
+    base, over, params = ORACLE_ONLY[name]
+    fmr, yaml_, _precision, _params, over0, _ = product_build.PROGRAMS[base]
     ov = dict(over0 or {})
     ov.update(over)
-    return compile_files(name, fmr, yaml_, "faithful", params, ov)
+    return compile_files(name, fmr, yaml_, "faithful", (), ov)
 
 
 def build_oracle(name: str) -> str:
     c = compiled_extra(name) if name in ORACLE_ONLY else compiled(name)
-    src = write_oracle(OUT, c.layout, c.mm)
+    params = ORACLE_ONLY[name][2] if name in ORACLE_ONLY else ()
+    src = write_oracle(OUT, c.layout, c.mm, params)
     out = oracle_path(name)
     cmd = ["gcc"] + CFLAGS + ["-I", OUT, "-I", product_build.MPI_STUB, src, "-o", out + ".tmp", "-lm"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

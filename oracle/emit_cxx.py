@@ -44,7 +44,17 @@ def _ctype(elem: str) -> str:
     return C_TYPES.get(elem, "double")
 
 
-def emit_cxx(lay: Layout, mm: MMProgram) -> str:
+def _param_nodes(mm: MMProgram, params) -> Dict[int, int]:
+    """OM node id -> parameter index for the named scalars made parameters
+    (same rule as the point IR: a non-grid node bound to that name)."""
+    out = {}
+    for nid, n in mm.om.step_graph.items():
+        if n.name in params and n.typ[0] == "elem":
+            out[nid] = list(params).index(n.name)
+    return out
+
+
+def emit_cxx(lay: Layout, mm: MMProgram, params=()) -> str:
     dim = lay.dim
     axes = lay.axes
     shape = lay.shape()
@@ -62,6 +72,16 @@ def emit_cxx(lay: Layout, mm: MMProgram) -> str:
         w(f"{t} {f}{dims_decl};")
         w(f"static {t} Next_{f}{dims_decl};")
     w("struct Formura_Navigator navi;")
+    pmap = _param_nodes(mm, params)
+    if params:
+        # per-run values of the named scalars (batched-launch parity tests)
+        w(f"static double fmr_param[{len(params)}];")
+        w("int Formura_Set_Params(const double *p, int np)")
+        w("{")
+        w(f"  if (np != {len(params)}) return -1;")
+        w(f"  for (int i = 0; i < {len(params)}; ++i) fmr_param[i] = p[i];")
+        w("  return 0;")
+        w("}")
     stores = {}
     om_nodes = []
     for k, n in mm.step.items():
@@ -126,6 +146,9 @@ def emit_cxx(lay: Layout, mm: MMProgram) -> str:
             inst = mn.inst
             ct = _ctype(mn.elem)
             tag = inst[0]
+            if mn.om_id in pmap and tag not in ("LoadCursorStatic", "LoadCursor", "Store"):
+                names[j] = f"fmr_param[{pmap[mn.om_id]}]"
+                continue
             if tag == "LoadCursorStatic":
                 names[j] = f"{inst[2]}{acc(inst[1])}"
                 continue
@@ -181,11 +204,11 @@ def emit_cxx(lay: Layout, mm: MMProgram) -> str:
     return "\n".join(out) + "\n"
 
 
-def write_oracle(outdir: str, lay: Layout, mm: MMProgram) -> str:
+def write_oracle(outdir: str, lay: Layout, mm: MMProgram, params=()) -> str:
     os.makedirs(outdir, exist_ok=True)
     with open(os.path.join(outdir, f"{lay.name}.h"), "w") as f:
         f.write(header_text(lay))
     path = os.path.join(outdir, f"{lay.name}_cpu.c")
     with open(path, "w") as f:
-        f.write(emit_cxx(lay, mm))
+        f.write(emit_cxx(lay, mm, params))
     return path

@@ -187,3 +187,31 @@ def test_reference_generated_c_equals_oracle_bitwise(tmp_path, reference_tree):
         assert np.array_equal(p.arrays["U"][2:602, 2:602], lib.interior("U"))
         assert np.array_equal(p.arrays["V"][2:602, 2:602], lib.interior("V"))
     assert lib.time_step == 40
+
+
+def test_parameterised_oracle_equals_baked_constants():
+    """The oracle with F, k, dt, cu, cv read at run time (used to check the
+    batched launches) equals the oracle with the literals baked in, when the
+    run-time values are computed with the program's own arithmetic."""
+    import ctypes
+    from formura_amd.batched import InstanceSpec
+    from oracle.build_oracle import build_oracle, compiled, compiled_extra, oracle_path
+    for n in ("gs2d", "gs2d_p256"):
+        if not os.path.exists(oracle_path(n)):
+            build_oracle(n)
+    baked = StepLibrary(oracle_path("gs2d"), compiled("gs2d").layout, device=False)
+    param = StepLibrary(oracle_path("gs2d_p256"), compiled_extra("gs2d_p256").layout, device=False)
+    baked.init()
+    param.init()
+    p = (ctypes.c_double * 5)(*InstanceSpec(256, 256, 0.04, 0.06).params())
+    assert param.lib.Formura_Set_Params(p, 5) == 0
+    u, v = gray_scott_ic((256, 256), 77)
+    baked.set_state({"U": u, "V": v})
+    param.set_state({"U": u, "V": v})
+    for _ in range(5):       # 5 x 32 steps
+        baked.forward()
+    for _ in range(4):       # 4 x 40 steps
+        param.forward()
+    assert baked.time_step == param.time_step == 160
+    a, b = param.get_state(), baked.get_state()
+    assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["V"], b["V"])
