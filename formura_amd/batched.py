@@ -191,3 +191,59 @@ def autotune(lib: BatchLibrary, bucket_of: Sequence[int], probe_steps: int = 24,
         plan[b] = best[1]
     lib.plan(bucket_of, plan)
     return plan, table
+
+
+def sweep_benchmark(steps: int = 5000, plain_steps: int = 200, seed: int = 1000, log=print) -> Dict:
+    """The batched sweep end to end on one GPU.
+
+    FP32 bulk batch (autotuned plan, hipGraph) and its FP64 twin advance all
+    256 instances by `steps`; the FP64 run is the accuracy reference (bit-equal
+    to the CPU oracle per instance, see the GPU tests). The plain method --
+    one FP64 single-site-per-thread launch per instance per step -- is timed
+    over `plain_steps` steps and scaled linearly to `steps` (each step of the
+    plain method is the same set of launches)."""
+    specs = sweep_instances()
+    buckets = size_buckets(specs)
+    cells = sum(s.nx * s.ny for s in specs)
+    ics = [initial_state(s, seed + i) for i, s in enumerate(specs)]
+    res: Dict = {"instances": len(specs), "cells": cells, "steps": steps}
+    libs = {}
+    for name in ("gs2d_batch_f32", "gs2d_batch_f64"):
+        lib = open_batch(name)
+        lib.init(specs)
+        t0 = time.perf_counter()
+        plan, table = autotune(lib, buckets)
+        log(f"{name}: plan {plan} (autotune {time.perf_counter() - t0:.1f} s)")
+        for i, (u, v) in enumerate(ics):
+            lib.set_state(i, [u, v])
+        lib.advance(8, graph=True)          # capture a small graph once (warm-up) ...
+        for i, (u, v) in enumerate(ics):    # ... and restart from the initial state
+            lib.set_state(i, [u, v])
+        t0 = time.perf_counter()
+        ms = lib.advance(steps, graph=True)
+        wall = time.perf_counter() - t0
+        libs[name] = lib
+        res[name] = {"plan": plan, "gpu_ms": round(ms, 2), "wall_s_incl_capture": round(wall, 3),
+                     "glups": round(steps * cells / (ms * 1e-3) / 1e9, 1)}
+        log(f"{name}: {res[name]}")
+    f32, f64 = libs["gs2d_batch_f32"], libs["gs2d_batch_f64"]
+    worst, mean = 0.0, 0.0
+    for i in range(len(specs)):
+        for f in range(2):
+            d, r, _ = f32.diff(i, f, f64)
+            worst = max(worst, d / r)
+            mean += d / r
+    res["fp32_vs_fp64_rel_l2_max"] = worst
+    res["fp32_vs_fp64_rel_l2_mean"] = mean / (2 * len(specs))
+    # plain FP64 method on the same instances
+    for i, (u, v) in enumerate(ics):
+        f64.set_state(i, [u, v])
+    f64.advance_plain(2)
+    pms = f64.advance_plain(plain_steps)
+    plain_ms = pms * steps / plain_steps
+    res["plain_fp64"] = {"gpu_ms_measured": round(pms, 2), "steps_measured": plain_steps,
+                         "gpu_ms_scaled": round(plain_ms, 1),
+                         "glups": round(plain_steps * cells / (pms * 1e-3) / 1e9, 1)}
+    res["speedup_batched_fp32_vs_plain_fp64"] = round(plain_ms / res["gs2d_batch_f32"]["gpu_ms"], 2)
+    res["speedup_batched_fp64_vs_plain_fp64"] = round(plain_ms / res["gs2d_batch_f64"]["gpu_ms"], 2)
+    return res
