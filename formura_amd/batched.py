@@ -231,6 +231,10 @@ def sweep_benchmark(steps: int = 5000, plain_steps: int = 200, seed: int = 1000,
     for i in range(len(specs)):
         for f in range(2):
             d, r, _ = f32.diff(i, f, f64)
+            if not r > 0:
+                h32, h64 = f32.get_state(i)[f], f64.get_state(i)[f]
+                raise RuntimeError(f"instance {i} field {f}: reference norm {r}, diff {d}; host sums "
+                                   f"{h32.sum()} {h64.sum()} bucket {buckets[i]} spec {specs[i]}")
             worst = max(worst, d / r)
             mean += d / r
     res["fp32_vs_fp64_rel_l2_max"] = worst

@@ -477,20 +477,17 @@ class BatchLibrary {
   float advance_plain(int steps) {
     FMR_CHECK(hipStreamSynchronize(stream));
     FMR_CHECK(hipEventRecord(ev0, stream));
-    for (int s = 0; s < steps; ++s)
+    for (int s = 0; s < steps; ++s) {
       for (int i = 0; i < n; ++i) {
-        int& par = buckets.empty() ? plain_parity : buckets[bucket_of[i]].parity;
         const size_t cnt = (size_t)host[i].nx * host[i].ny;
         const int blocks = (int)((cnt + 255) / 256 < 65536 ? (cnt + 255) / 256 : 65536);
-        hipLaunchKernelGGL((batched_direct_kernel<P>), dim3(blocks), dim3(256), 0, stream, d_inst, i, par);
-        (void)par;
+        hipLaunchKernelGGL((batched_direct_kernel<P>), dim3(blocks), dim3(256), 0, stream, d_inst, i,
+                           parity_of(i));
       }
-    FMR_CHECK(hipGetLastError());
-    // every instance advanced `steps` times: flip each bucket's parity once per step
-    if (buckets.empty()) {
-      plain_parity ^= (steps & 1);
-    } else {
-      for (auto& bk : buckets) bk.parity ^= (steps & 1);
+      FMR_CHECK(hipGetLastError());
+      // every instance advanced one step: flip every parity
+      if (buckets.empty()) plain_parity ^= 1;
+      for (auto& bk : buckets) bk.parity ^= 1;
     }
     FMR_CHECK(hipEventRecord(ev1, stream));
     FMR_CHECK(hipEventSynchronize(ev1));

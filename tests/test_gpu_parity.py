@@ -134,3 +134,43 @@ def test_fp64_reductions_match_numpy():
     u = st["u"]
     assert abs(s - u.sum()) <= 1e-9 * np.abs(u).sum()
     assert abs(s2 - (u * u).sum()) <= 1e-12 * (u * u).sum()
+
+
+def test_device_diff_matches_host():
+    a = hip("heat_t48_f32")
+    b = hip("heat_t48")
+    st = heat_state(b, 21)
+    for lib in (a, b):
+        lib.set_state(st)
+        lib.upload()
+        lib.forward_device(5)
+        lib.download()
+    d, r, m = a.diff(0, b)
+    ga, gb = a.get_state()["u"], b.get_state()["u"]
+    assert abs(r - np.linalg.norm(gb)) <= 1e-12 * np.linalg.norm(gb)
+    assert abs(d - np.linalg.norm(ga - gb)) <= 1e-9 * max(np.linalg.norm(ga - gb), 1e-300)
+    assert m == np.abs(ga - gb).max()
+
+
+def test_grayscott512_fp32_bulk_vs_fp64_10000_steps():
+    """Gray-Scott 3D 512^3, dx = 1/512, dt = 0.1 dx^2/Du, 10000 steps: the FP32
+    bulk library against the FP64 library on the GPU (the FP64 library is
+    bit-equal to the CPU oracle on the same program, see the tests above and
+    the 64^3 case).  FP64 only in the reductions: sums of u and v and the L2
+    norms.  Bound: relative L2 error <= 1e-3 per field."""
+    f64 = hip("gs3d512")
+    f32 = hip("gs3d512_f32")
+    st = gs_state(f64, 512)
+    for lib in (f64, f32):
+        lib.set_state(st)
+        lib.upload()
+    s0 = [f64.reduce(f)[0] for f in range(2)]
+    _say("[parity] gs3d512 FP64 and FP32, 10000 steps each")
+    f64.forward_device(10000)
+    f32.forward_device(10000)
+    for f in range(2):
+        d, r, m = f32.diff(f, f64)
+        s64, s32 = f64.reduce(f)[0], f32.reduce(f)[0]
+        _say(f"[parity] field {f}: rel L2 {d / r:.3e} max abs {m:.3e}; sum FP64 {s64:.10e} "
+             f"FP32 {s32:.10e} (initial {s0[f]:.10e})")
+        assert d / r <= 1e-3
