@@ -156,6 +156,16 @@ class BatchLibrary:
         return math.sqrt(r[0]), math.sqrt(r[1]), r[2]
 
 
+def relative_l2(diff_result, n: int, floor: float = 1e-6) -> float:
+    """||a - b|| / max(||b||, floor * sqrt(n)) from a device diff (d, r, max).
+
+    A field whose RMS has decayed below `floor` (a Gray-Scott pattern that died
+    out: v -> 0, whose FP64 squares underflow) is compared on the absolute
+    scale of an O(1) field instead of its own vanishing norm."""
+    d, r, _ = diff_result
+    return d / max(r, floor * math.sqrt(n))
+
+
 def size_buckets(specs: Sequence[InstanceSpec], edges=(256, 512)) -> List[int]:
     """Bucket of each instance by side length: [128,256), [256,512), [512,1024]."""
     return [sum(1 for e in edges if s.ny >= e) for s in specs]
@@ -228,15 +238,11 @@ def sweep_benchmark(steps: int = 5000, plain_steps: int = 200, seed: int = 1000,
         log(f"{name}: {res[name]}")
     f32, f64 = libs["gs2d_batch_f32"], libs["gs2d_batch_f64"]
     worst, mean = 0.0, 0.0
-    for i in range(len(specs)):
+    for i, s in enumerate(specs):
         for f in range(2):
-            d, r, _ = f32.diff(i, f, f64)
-            if not r > 0:
-                h32, h64 = f32.get_state(i)[f], f64.get_state(i)[f]
-                raise RuntimeError(f"instance {i} field {f}: reference norm {r}, diff {d}; host sums "
-                                   f"{h32.sum()} {h64.sum()} bucket {buckets[i]} spec {specs[i]}")
-            worst = max(worst, d / r)
-            mean += d / r
+            worst_i = relative_l2(f32.diff(i, f, f64), s.nx * s.ny)
+            worst = max(worst, worst_i)
+            mean += worst_i
     res["fp32_vs_fp64_rel_l2_max"] = worst
     res["fp32_vs_fp64_rel_l2_mean"] = mean / (2 * len(specs))
     # plain FP64 method on the same instances
