@@ -1,0 +1,617 @@
+// surrogate.hip -- learned surrogate for periodic 2D Gray-Scott: a residual
+// CNN of five circular 3x3 convolutions (4 -> 48 -> 48 -> 48 -> 48 -> 2, GELU
+// between them) that predicts the state 32 stencil steps ahead, with
+// hand-written CDNA4 kernels for the forward pass, the backward pass (data and
+// weight gradients), the loss and Adam.
+//
+//   x0  = [ (u-0.5)*2, v*4, (F-0.0375)/0.0175, (k-0.0575)/0.0075 ]   (per pixel)
+//   a_l = gelu(conv_l(a_{l-1}) + b_l),  l = 1..4     (a_0 = x0)
+//   z5  = conv_5(a_4) + b_5
+//   [u,v](t+32) = [u,v](t) + s .* z5                (s: per-channel output scale)
+//
+// Numerics: weights and activations are BF16 (round to nearest even) on MFMA
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; bias, GELU, residual,
+// loss and Adam in FP32 (loss reduced in FP64).  Layouts: activations NHWC
+// BF16 [B][H][W][C] with C padded to a multiple of 8; the state planes FP32
+// [B][2][H][W]; convolution weights BF16 [Cout][3][3][Cin] for the forward
+// pass and [Cin][3][3][Cout] for the data gradient; master weights FP32.
+//
+// Convolution as implicit GEMM on MFMA 16x16x32: M = pixels, N = output
+// channels, K = 9 * input channels ordered (dy, dx, c).  MFMA lane layout
+// (wave64): lane l holds A[m = l%16][k = 8*(l/16) + j] and B[k = 8*(l/16) + j]
+// [n = l%16] for j = 0..7, and C[m = 4*(l/16) + i][n = l%16] for i = 0..3.
+// Eight consecutive k are eight consecutive channels of one tap, so an A or B
+// fragment is one 16-byte load.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#define SCHECK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      fprintf(stderr, "surrogate: HIP error '%s' at %s:%d (%s)\n", hipGetErrorString(e_), \
+              __FILE__, __LINE__, #expr);                                             \
+      abort();                                                                        \
+    }                                                                                 \
+  } while (0)
+
+namespace sur {
+
+typedef unsigned short bf16;
+typedef short bf16x8 __attribute__((ext_vector_type(8)));        // 8 BF16 as raw bits
+typedef __bf16 bf16x8_mfma __attribute__((ext_vector_type(8)));  // the MFMA operand type
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_mfma, a),
+                                                 __builtin_bit_cast(bf16x8_mfma, b), c, 0, 0, 0);
+}
+
+constexpr int NL = 5;
+constexpr int CIN[NL] = {4, 48, 48, 48, 48};
+constexpr int COUT[NL] = {48, 48, 48, 48, 2};
+constexpr int CP_IN[NL] = {8, 48, 48, 48, 48};      // input channels padded to 8
+constexpr int CP_OUT[NL] = {48, 48, 48, 48, 8};     // output channels padded (layer 5: 8)
+constexpr int NT_OUT[NL] = {3, 3, 3, 3, 1};         // 16-wide N tiles of the forward GEMM
+
+__host__ __device__ inline bf16 f2bf(float f) {   // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16 h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+  return cdf + z * pdf;
+}
+
+__device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// ---------------------------------------------------------------------------
+// input planes -> x0 (NHWC, 8 channels, channels 4..7 zero)
+__global__ void pack_input(const float* __restrict__ state, const float* __restrict__ fk, bf16* __restrict__ x0,
+                           int B, int H, int W) {
+  const size_t n = (size_t)B * H * W;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(p / ((size_t)H * W));
+    const size_t hw = p % ((size_t)H * W);
+    const float u = state[((size_t)b * 2 + 0) * H * W + hw];
+    const float v = state[((size_t)b * 2 + 1) * H * W + hw];
+    bf16* o = x0 + p * 8;
+    o[0] = f2bf((u - 0.5f) * 2.0f);
+    o[1] = f2bf(v * 4.0f);
+    o[2] = f2bf((fk[2 * b] - 0.0375f) / 0.0175f);
+    o[3] = f2bf((fk[2 * b + 1] - 0.0575f) / 0.0075f);
+    o[4] = o[5] = o[6] = o[7] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Implicit-GEMM convolution on MFMA.  One wave computes 64 consecutive pixels
+// of one image row (4 M tiles) times NT output tiles of 16 channels.
+//   DIR = +1: forward,       out[p][n] = sum_{d,c} wt[n][d][c] * in[p + d - 1][c]
+//   DIR = -1: data gradient, out[p][n] = sum_{d,c} wt[n][d][c] * in[p - d + 1][c]
+// `in` has CI channels (multiple of 8), `wt` is [NT*16 padded n][9][CI].
+// Epilogues:
+//   EPI_HIDDEN: z = acc + bias; zq = bf16(z) -> zout; a = bf16(gelu(z)) -> aout
+//   EPI_OUT:    state_out = state_in + scale * (acc + bias) for n < 2
+//   EPI_DGELU:  aout = bf16(acc * gelu'(zq[p][n]))   (dz of the previous layer)
+enum { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_DGELU = 2 };
+
+template <int CI, int NT, int DIR, int EPI>
+__global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ wt,
+                                                 const float* __restrict__ bias, bf16* __restrict__ zout,
+                                                 bf16* __restrict__ aout, const bf16* __restrict__ zq,
+                                                 const float* __restrict__ state_in, float* __restrict__ state_out,
+                                                 const float* __restrict__ scale, int B, int H, int W, int CO) {
+  constexpr int KSTEPS = (9 * CI + 31) / 32;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int segs = W / 64;
+  const int job = blockIdx.x * 4 + wave;           // (b, y, segment)
+  if (job >= B * H * segs) return;
+  const int b = job / (H * segs);
+  const int y = (job / segs) % H;
+  const int x0 = (job % segs) * 64;
+  const int m = lane % 16, g = lane / 16;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[t][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const size_t img = (size_t)b * H * W;
+#pragma unroll 2
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int k = ks * 32 + g * 8;                 // first of this lane's 8 k
+    const bool kvalid = k < 9 * CI;
+    const int tap = kvalid ? k / CI : 0;
+    const int c = k % CI;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int yy = wrapi(y + DIR * dy, H);
+    bf16x8 bfr[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int n = nt * 16 + m;
+      bfr[nt] = kvalid ? *(const bf16x8*)(wt + ((size_t)n * 9 + tap) * CI + c) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int xx = wrapi(x0 + t * 16 + m + DIR * dx, W);
+      const bf16x8 afr = kvalid ? *(const bf16x8*)(in + (img + (size_t)yy * W + xx) * CI + c)
+                                : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[t][nt] = mfma_bf16(afr, bfr[nt], acc[t][nt]);
+    }
+  }
+  // epilogue: lane holds pixels x0 + t*16 + 4*g + i, channel nt*16 + m
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int n = nt * 16 + m;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + t * 16 + 4 * g + i;
+        const size_t p = img + (size_t)y * W + x;
+        const float s = acc[t][nt][i];
+        if constexpr (EPI == EPI_HIDDEN) {
+          const float z = s + bias[n];
+          zout[p * CO + n] = f2bf(z);
+          aout[p * CO + n] = f2bf(gelu(z));
+        } else if constexpr (EPI == EPI_OUT) {
+          if (n < 2) {
+            const size_t q = ((size_t)b * 2 + n) * H * W + (size_t)y * W + x;
+            state_out[q] = state_in[q] + scale[n] * (s + bias[n]);
+          }
+        } else {
+          aout[p * CO + n] = f2bf(s * gelu_grad(bf2f(zq[p * CO + n])));
+        }
+      }
+    }
+}
+
+// loss and the output gradient: dz5[p][n] = 2 s_n (out - target) / count for
+// n < 2 (BF16, 8 channels), per-block FP64 partial sums of (out - target)^2
+__global__ void __launch_bounds__(256) loss_grad(const float* __restrict__ out, const float* __restrict__ target,
+                                                 const float* __restrict__ scale, bf16* __restrict__ dz5,
+                                                 double* __restrict__ part, int B, int H, int W) {
+  __shared__ double red[256];
+  const size_t n = (size_t)B * H * W;
+  const double inv = 1.0 / (double)(n * 2);
+  double acc = 0.0;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(p / ((size_t)H * W));
+    const size_t hw = p % ((size_t)H * W);
+    bf16* o = dz5 + p * 8;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const size_t q = ((size_t)b * 2 + c) * H * W + hw;
+      const float d = out[q] - target[q];
+      acc += (double)d * d;
+      o[c] = f2bf((float)(2.0 * inv) * d * scale[c]);
+    }
+#pragma unroll
+    for (int c = 2; c < 8; ++c) o[c] = 0;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] * inv;
+}
+
+// Weight gradient on MFMA: dW[n][tap][c] = sum_p dz[p][n] * a[p + d(tap)][c].
+// M = output channel n (16-tiles), N = (tap, c) (16-tiles), K = pixels.  One
+// wave owns one N tile for one pixel chunk (rows [r0, r0 + RC) of one image)
+// and writes FP32 partial sums part[chunk][n][tap*CI + c].
+template <int CI, int CO, int MT>
+__global__ void __launch_bounds__(256) wgrad_mfma(const bf16* __restrict__ a, const bf16* __restrict__ dz,
+                                                  float* __restrict__ part, int B, int H, int W, int RC) {
+  constexpr int NCOL = 9 * CI;
+  constexpr int NTILES = (NCOL + 15) / 16;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int job = blockIdx.x * 4 + wave;
+  const int chunks = B * (H / RC);
+  if (job >= chunks * NTILES) return;
+  const int ntile = job % NTILES;
+  const int chunk = job / NTILES;
+  const int b = chunk / (H / RC);
+  const int r0 = (chunk % (H / RC)) * RC;
+  const int m = lane % 16, g = lane / 16;
+  const int col = ntile * 16 + m;                   // this lane's B column (tap, c)
+  const bool col_ok = col < NCOL;
+  const int tap = col_ok ? col / CI : 0, c = col % CI;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  f32x4 acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const size_t img = (size_t)b * H * W;
+  for (int y = r0; y < r0 + RC; ++y) {
+    const int ya = wrapi(y + dy, H);
+    for (int xk = 0; xk < W; xk += 32) {
+      const int xs = xk + 8 * g;                    // this lane's 8 pixels: xs .. xs+7
+      bf16x8 bfr;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int xa = wrapi(xs + j + dx, W);
+        bfr[j] = col_ok ? (short)a[(img + (size_t)ya * W + xa) * CI + c] : (short)0;
+      }
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int n = t * 16 + m;
+        bf16x8 afr;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          afr[j] = n < CO ? (short)dz[(img + (size_t)y * W + xs + j) * (CO < 8 ? 8 : CO) + n] : (short)0;
+        acc[t] = mfma_bf16(afr, bfr, acc[t]);
+      }
+    }
+  }
+  // C[m = output channel 4g+i of tile t][n = column col]
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int n = t * 16 + 4 * g + i;
+      if (n < CO && col_ok) part[((size_t)chunk * CO + n) * NCOL + col] = acc[t][i];
+    }
+}
+
+// bias gradient partials: part[chunk][n] = sum over the chunk's pixels of dz[p][n]
+template <int CO>
+__global__ void __launch_bounds__(64) bgrad(const bf16* __restrict__ dz, float* __restrict__ part, int B, int H,
+                                            int W, int RC) {
+  constexpr int CS = CO < 8 ? 8 : CO;
+  const int chunk = blockIdx.x;
+  const int b = chunk / (H / RC);
+  const int r0 = (chunk % (H / RC)) * RC;
+  for (int n = threadIdx.x; n < CO; n += 64) {
+    float s = 0.f;
+    for (int y = r0; y < r0 + RC; ++y)
+      for (int x = 0; x < W; ++x) s += bf2f(dz[(((size_t)b * H + y) * W + x) * CS + n]);
+    part[(size_t)chunk * CO + n] = s;
+  }
+}
+
+// sum partials over chunks in a fixed order: out[i] = sum_c part[c * n + i]
+__global__ void sum_chunks(const float* __restrict__ part, float* __restrict__ out, int chunks, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += part[(size_t)c * n + i];
+    out[i] = s;
+  }
+}
+
+// Adam on the FP32 master parameters
+__global__ void adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
+                     float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    w[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+  }
+}
+
+// master [co][tap][ci] FP32 -> forward BF16 [co_pad][tap][ci_pad] and data-gradient
+// BF16 [ci_pad_n][tap][co_pad_k] (zero padded)
+__global__ void cast_weights(const float* __restrict__ w, bf16* __restrict__ wf, bf16* __restrict__ wb, int ci,
+                             int co, int cif, int cof, int cib, int cob) {
+  const int n = cof * 9 * cif;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = i % cif, tap = (i / cif) % 9, o = i / (cif * 9);
+    wf[i] = (o < co && c < ci) ? f2bf(w[((size_t)o * 9 + tap) * ci + c]) : (bf16)0;
+  }
+  const int nb = cib * 9 * cob;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+    const int o = i % cob, tap = (i / cob) % 9, c = i / (cob * 9);
+    wb[i] = (o < co && c < ci) ? f2bf(w[((size_t)o * 9 + tap) * ci + c]) : (bf16)0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct Net {
+  int H = 0, W = 0, maxB = 0;
+  int RC = 16;                      // rows per weight-gradient chunk
+  // parameters: per layer W [co][9][ci] then b [co], FP32, contiguous
+  int off_w[NL], off_b[NL], nparam = 0;
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  bf16 *wf[NL], *wb[NL];
+  float* scale = nullptr;           // output scale s (2)
+  bf16 *x0 = nullptr, *z[4], *a[4], *dz5 = nullptr, *dza = nullptr, *dzb = nullptr;
+  float *part = nullptr, *out = nullptr;
+  double* lpart = nullptr;
+  size_t part_floats = 0;
+  hipStream_t stream = nullptr;
+  int adam_t = 0;
+  bool ready = false;
+};
+
+Net net;
+
+int grid_for(size_t n, int threads) {
+  size_t b = (n + threads - 1) / threads;
+  return (int)(b < 65536 ? b : 65536);
+}
+
+void cast_all() {
+  for (int l = 0; l < NL; ++l) {
+    const int cob = (l < NL - 1) ? 48 : 8;    // data-gradient K channels (layer 5: dz has 8)
+    const int cib = 48;                       // data-gradient N (input channels, padded to 3 tiles)
+    hipLaunchKernelGGL(cast_weights, dim3(64), dim3(256), 0, net.stream, net.w + net.off_w[l], net.wf[l], net.wb[l],
+                       CIN[l], COUT[l], CP_IN[l], NT_OUT[l] * 16, cib, cob);
+  }
+  SCHECK(hipGetLastError());
+}
+
+void forward(const float* state, const float* fk, float* out_state, int B) {
+  const int H = net.H, W = net.W;
+  const size_t px = (size_t)B * H * W;
+  hipLaunchKernelGGL(pack_input, dim3(grid_for(px, 256)), dim3(256), 0, net.stream, state, fk, net.x0, B, H, W);
+  const int blocks = (int)((B * H * (W / 64) + 3) / 4);
+  const bf16* in = net.x0;
+  hipLaunchKernelGGL((conv_mfma<8, 3, 1, EPI_HIDDEN>), dim3(blocks), dim3(256), 0, net.stream, in, net.wf[0],
+                     net.w + net.off_b[0], net.z[0], net.a[0], nullptr, nullptr, nullptr, nullptr, B, H, W, 48);
+  for (int l = 1; l < 4; ++l)
+    hipLaunchKernelGGL((conv_mfma<48, 3, 1, EPI_HIDDEN>), dim3(blocks), dim3(256), 0, net.stream, net.a[l - 1],
+                       net.wf[l], net.w + net.off_b[l], net.z[l], net.a[l], nullptr, nullptr, nullptr, nullptr, B, H,
+                       W, 48);
+  hipLaunchKernelGGL((conv_mfma<48, 1, 1, EPI_OUT>), dim3(blocks), dim3(256), 0, net.stream, net.a[3], net.wf[4],
+                     net.w + net.off_b[4], nullptr, nullptr, nullptr, state, out_state, net.scale, B, H, W, 8);
+  SCHECK(hipGetLastError());
+}
+
+// gradients of the loss (out vs target) into net.g; returns the loss
+double backward(const float* target, int B) {
+  const int H = net.H, W = net.W;
+  const size_t px = (size_t)B * H * W;
+  const int lblocks = 1024;
+  hipLaunchKernelGGL(loss_grad, dim3(lblocks), dim3(256), 0, net.stream, net.out, target, net.scale, net.dz5,
+                     net.lpart, B, H, W);
+  const int chunks = B * (H / net.RC);
+  const int blocks = (int)((B * H * (W / 64) + 3) / 4);
+  // layer 5: weights from a4 and dz5; data gradient -> dz4
+  const bf16* dz = net.dz5;
+  bf16* dzn = net.dza;
+  for (int l = NL - 1; l >= 0; --l) {
+    const bf16* ain = l == 0 ? net.x0 : net.a[l - 1];
+    const int ncol = 9 * CIN[l];
+    const int ci = CP_IN[l];
+    // weight-gradient partials
+    const int ntiles = (9 * ci + 15) / 16;
+    const int wblocks = (chunks * ntiles + 3) / 4;
+    if (l == NL - 1)
+      hipLaunchKernelGGL((wgrad_mfma<48, 2, 1>), dim3(wblocks), dim3(256), 0, net.stream, ain, dz, net.part, B, H, W,
+                         net.RC);
+    else if (l == 0)
+      hipLaunchKernelGGL((wgrad_mfma<8, 48, 3>), dim3(wblocks), dim3(256), 0, net.stream, ain, dz, net.part, B, H, W,
+                         net.RC);
+    else
+      hipLaunchKernelGGL((wgrad_mfma<48, 48, 3>), dim3(wblocks), dim3(256), 0, net.stream, ain, dz, net.part, B, H,
+                         W, net.RC);
+    // the first layer's partials hold columns for 8 padded input channels: sum
+    // them, then compact to the 4 real ones on the host side of the layout
+    const int ncol_pad = 9 * ci;
+    hipLaunchKernelGGL(sum_chunks, dim3(grid_for((size_t)COUT[l] * ncol_pad, 256)), dim3(256), 0, net.stream,
+                       net.part, net.part + (size_t)chunks * COUT[l] * ncol_pad, chunks, COUT[l] * ncol_pad);
+    if (ci == CIN[l]) {
+      SCHECK(hipMemcpyAsync(net.g + net.off_w[l], net.part + (size_t)chunks * COUT[l] * ncol_pad,
+                            sizeof(float) * COUT[l] * ncol, hipMemcpyDeviceToDevice, net.stream));
+    } else {
+      SCHECK(hipMemcpy2DAsync(net.g + net.off_w[l], sizeof(float) * CIN[l],
+                              net.part + (size_t)chunks * COUT[l] * ncol_pad, sizeof(float) * ci,
+                              sizeof(float) * CIN[l], (size_t)COUT[l] * 9, hipMemcpyDeviceToDevice, net.stream));
+    }
+    // bias gradient
+    if (COUT[l] == 2)
+      hipLaunchKernelGGL((bgrad<2>), dim3(chunks), dim3(64), 0, net.stream, dz, net.part, B, H, W, net.RC);
+    else
+      hipLaunchKernelGGL((bgrad<48>), dim3(chunks), dim3(64), 0, net.stream, dz, net.part, B, H, W, net.RC);
+    hipLaunchKernelGGL(sum_chunks, dim3(1), dim3(64), 0, net.stream, net.part, net.g + net.off_b[l], chunks,
+                       COUT[l]);
+    // data gradient into the previous layer (times gelu')
+    if (l > 0) {
+      if (l == NL - 1)
+        hipLaunchKernelGGL((conv_mfma<8, 3, -1, EPI_DGELU>), dim3(blocks), dim3(256), 0, net.stream, dz, net.wb[l],
+                           nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr, nullptr, B, H, W, 48);
+      else
+        hipLaunchKernelGGL((conv_mfma<48, 3, -1, EPI_DGELU>), dim3(blocks), dim3(256), 0, net.stream, dz,
+                           net.wb[l], nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr, nullptr, B, H, W, 48);
+      dz = dzn;
+      dzn = (dzn == net.dza) ? net.dzb : net.dza;
+    }
+    (void)px;
+  }
+  SCHECK(hipGetLastError());
+  std::vector<double> lp(lblocks);
+  SCHECK(hipMemcpyAsync(lp.data(), net.lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  double loss = 0.0;
+  for (double v : lp) loss += v;
+  return loss;
+}
+
+}  // namespace sur
+
+using namespace sur;
+
+extern "C" {
+
+int Sur_Num_Params(void) {
+  int n = 0;
+  for (int l = 0; l < NL; ++l) n += COUT[l] * 9 * CIN[l] + COUT[l];
+  return n;
+}
+
+// H, W: image size (W a multiple of 64, H a multiple of 16); maxB: largest batch
+int Sur_Init(int H, int W, int maxB) {
+  if (net.ready || W % 64 != 0 || H % 16 != 0 || maxB < 1) return -1;
+  net.H = H;
+  net.W = W;
+  net.maxB = maxB;
+  SCHECK(hipStreamCreateWithFlags(&net.stream, hipStreamNonBlocking));
+  int off = 0;
+  for (int l = 0; l < NL; ++l) {
+    net.off_w[l] = off;
+    off += COUT[l] * 9 * CIN[l];
+    net.off_b[l] = off;
+    off += COUT[l];
+  }
+  net.nparam = off;
+  for (float** p : {&net.w, &net.g, &net.m1, &net.m2}) {
+    SCHECK(hipMalloc(p, sizeof(float) * off));
+    SCHECK(hipMemsetAsync(*p, 0, sizeof(float) * off, net.stream));
+  }
+  for (int l = 0; l < NL; ++l) {
+    SCHECK(hipMalloc(&net.wf[l], sizeof(bf16) * NT_OUT[l] * 16 * 9 * CP_IN[l]));
+    SCHECK(hipMalloc(&net.wb[l], sizeof(bf16) * 48 * 9 * 48));
+  }
+  SCHECK(hipMalloc(&net.scale, sizeof(float) * 2));
+  const size_t px = (size_t)maxB * H * W;
+  SCHECK(hipMalloc(&net.x0, sizeof(bf16) * px * 8));
+  for (int l = 0; l < 4; ++l) {
+    SCHECK(hipMalloc(&net.z[l], sizeof(bf16) * px * 48));
+    SCHECK(hipMalloc(&net.a[l], sizeof(bf16) * px * 48));
+  }
+  SCHECK(hipMalloc(&net.dz5, sizeof(bf16) * px * 8));
+  SCHECK(hipMalloc(&net.dza, sizeof(bf16) * px * 48));
+  SCHECK(hipMalloc(&net.dzb, sizeof(bf16) * px * 48));
+  SCHECK(hipMalloc(&net.out, sizeof(float) * px * 2));
+  const int chunks = maxB * (H / net.RC);
+  net.part_floats = (size_t)chunks * 48 * 9 * 48 + (size_t)48 * 9 * 48;
+  SCHECK(hipMalloc(&net.part, sizeof(float) * net.part_floats));
+  SCHECK(hipMalloc(&net.lpart, sizeof(double) * 1024));
+  const float one[2] = {1.f, 1.f};
+  SCHECK(hipMemcpyAsync(net.scale, one, sizeof(one), hipMemcpyHostToDevice, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  net.ready = true;
+  return 0;
+}
+
+int Sur_Set_Params(const float* host) {
+  if (!net.ready) return -1;
+  SCHECK(hipMemcpyAsync(net.w, host, sizeof(float) * net.nparam, hipMemcpyHostToDevice, net.stream));
+  cast_all();
+  SCHECK(hipMemsetAsync(net.m1, 0, sizeof(float) * net.nparam, net.stream));
+  SCHECK(hipMemsetAsync(net.m2, 0, sizeof(float) * net.nparam, net.stream));
+  net.adam_t = 0;
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+int Sur_Get_Params(float* host) {
+  if (!net.ready) return -1;
+  SCHECK(hipMemcpyAsync(host, net.w, sizeof(float) * net.nparam, hipMemcpyDeviceToHost, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+int Sur_Get_Grads(float* host) {
+  if (!net.ready) return -1;
+  SCHECK(hipMemcpyAsync(host, net.g, sizeof(float) * net.nparam, hipMemcpyDeviceToHost, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+int Sur_Set_Scale(const float* s2) {
+  if (!net.ready) return -1;
+  SCHECK(hipMemcpyAsync(net.scale, s2, sizeof(float) * 2, hipMemcpyHostToDevice, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+// state, fk, out: device pointers. state/out [B][2][H][W] FP32, fk [B][2] FP32
+int Sur_Forward(const float* state, const float* fk, float* out, int B) {
+  if (!net.ready || B < 1 || B > net.maxB) return -1;
+  forward(state, fk, out, B);
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+// forward + loss + gradients (no update); loss written to *loss
+int Sur_Gradients(const float* state, const float* fk, const float* target, int B, double* loss) {
+  if (!net.ready || B < 1 || B > net.maxB) return -1;
+  forward(state, fk, net.out, B);
+  *loss = backward(target, B);
+  return 0;
+}
+
+// one Adam step (gradients from Sur_Gradients or computed here)
+int Sur_Train_Step(const float* state, const float* fk, const float* target, int B, float lr, double* loss) {
+  if (!net.ready || B < 1 || B > net.maxB) return -1;
+  forward(state, fk, net.out, B);
+  *loss = backward(target, B);
+  net.adam_t += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)net.adam_t), c2 = 1.f - powf(b2, (float)net.adam_t);
+  hipLaunchKernelGGL(adam, dim3(grid_for(net.nparam, 256)), dim3(256), 0, net.stream, net.w, net.g, net.m1, net.m2,
+                     net.nparam, lr, b1, b2, 1e-8f, c1, c2);
+  cast_all();
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+// device memory helpers, so a caller needs no other GPU runtime binding
+void* Sur_Alloc(size_t bytes) {
+  void* p = nullptr;
+  SCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+int Sur_Free(void* p) {
+  SCHECK(hipFree(p));
+  return 0;
+}
+int Sur_Copy(void* dst, const void* src, size_t bytes, int kind) {   // 0 H2D, 1 D2H, 2 D2D
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost
+                                                                        : hipMemcpyDeviceToDevice);
+  SCHECK(hipMemcpy(dst, src, bytes, k));
+  return 0;
+}
+
+// dst[i * n_each .. (i+1) * n_each) = *ptrs[i] for i < nptr (device pointers
+// listed in host memory); one launch-free copy per pointer on the library stream
+int Sur_Gather(const unsigned long long* host_ptrs, float* dst, int nptr, size_t n_each) {
+  for (int i = 0; i < nptr; ++i)
+    SCHECK(hipMemcpyAsync(dst + (size_t)i * n_each, (const void*)host_ptrs[i], n_each * sizeof(float),
+                          hipMemcpyDeviceToDevice, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+// time `reps` forward passes of batch B on device buffers; returns ms (events)
+float Sur_Time_Forward(const float* state, const float* fk, float* out, int B, int reps) {
+  if (!net.ready || B < 1 || B > net.maxB) return -1.f;
+  hipEvent_t e0, e1;
+  SCHECK(hipEventCreate(&e0));
+  SCHECK(hipEventCreate(&e1));
+  SCHECK(hipEventRecord(e0, net.stream));
+  for (int r = 0; r < reps; ++r) forward(state, fk, out, B);
+  SCHECK(hipEventRecord(e1, net.stream));
+  SCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SCHECK(hipEventElapsedTime(&ms, e0, e1));
+  SCHECK(hipEventDestroy(e0));
+  SCHECK(hipEventDestroy(e1));
+  return ms;
+}
+
+}  // extern "C"

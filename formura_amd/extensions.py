@@ -1,7 +1,7 @@
 """Libraries beyond the per-program step libraries: the batched 2D
 Gray-Scott library (FP32 bulk and its FP64 twin) built from
-examples/gray-scott-2d.fmr with F, k, dt, cu, cv as per-instance parameters.
-`build_all` compiles them in-tree."""
+examples/gray-scott-2d.fmr with F, k, dt, cu, cv as per-instance parameters,
+and the surrogate CNN library.  `build_all` compiles them in-tree."""
 from __future__ import annotations
 
 import os
@@ -28,8 +28,13 @@ def generate_batch(name: str) -> str:
     return src
 
 
+def build_surrogate() -> str:
+    """The surrogate CNN library (hand-written, not generated)."""
+    return _b.compile_hip(os.path.join(_b.CSRC, "surrogate.hip"), os.path.join(_b.BUILD, "libsurrogate.so"))
+
+
 def build_all() -> Dict[str, str]:
-    out = {}
+    out = {"surrogate": build_surrogate()}
     for name in BATCH_PROGRAMS:
         src = generate_batch(name)
         out[name] = _b.compile_hip(src, _b.lib_path(name))

@@ -1,0 +1,290 @@
+"""Learned surrogate for periodic 2D Gray-Scott on 256^2 (F in [0.02, 0.055],
+k in [0.05, 0.065]): a residual CNN that advances (u, v) by 32 stencil steps
+in one forward pass, trained on the GPU by the project's own kernels
+(formura_amd/csrc/surrogate.hip: BF16 MFMA forward and backward, Adam) on
+trajectories generated by the batched FP32 stencil (formura_amd/batched.py),
+and used with checking: every 8th super-step is also computed by the FP32
+stencil from the same state, and when the relative L2 difference exceeds the
+bound (1e-3) the stencil takes over for the rest of the rollout.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import build as _build
+from .batched import InstanceSpec, open_batch
+
+H = W = 256
+SUPER = 32            # stencil steps per forward pass
+CHECK_EVERY = 8       # super-steps between checks
+BOUND = 1e-3          # relative L2 bound of a checked super-step
+F_RANGE = (0.02, 0.055)
+K_RANGE = (0.05, 0.065)
+CIN = (4, 48, 48, 48, 48)
+COUT = (48, 48, 48, 48, 2)
+
+
+def num_params() -> int:
+    return sum(co * 9 * ci + co for ci, co in zip(CIN, COUT))
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights, zero biases; the last layer scaled by 0.1 so the
+    untrained network predicts almost no change."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (ci, co) in enumerate(zip(CIN, COUT)):
+        std = math.sqrt(2.0 / (9 * ci)) * (0.1 if l == len(CIN) - 1 else 1.0)
+        parts.append(rng.normal(0.0, std, size=co * 9 * ci))
+        parts.append(np.zeros(co))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class DeviceBuffer:
+    def __init__(self, lib, nbytes: int):
+        self.lib = lib
+        self.nbytes = nbytes
+        self.ptr = int(lib.Sur_Alloc(ctypes.c_size_t(nbytes)))
+        if not self.ptr:
+            raise MemoryError("device allocation failed")
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.lib.Sur_Copy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes), 0)
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        self.lib.Sur_Copy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), ctypes.c_size_t(out.nbytes), 1)
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.lib.Sur_Free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+
+class Surrogate:
+    """The CNN library (one per process; its weights and buffers are global)."""
+
+    _instance: Optional["Surrogate"] = None
+
+    def __init__(self, max_batch: int = 16):
+        path = os.path.join(_build.BUILD, "libsurrogate.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (no CPU fallback exists)")
+        L = self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        vp, fp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+        L.Sur_Alloc.argtypes = [ctypes.c_size_t]
+        L.Sur_Alloc.restype = vp
+        L.Sur_Free.argtypes = [vp]
+        L.Sur_Copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        L.Sur_Gather.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), vp, ctypes.c_int, ctypes.c_size_t]
+        L.Sur_Set_Params.argtypes = [fp]
+        L.Sur_Get_Params.argtypes = [fp]
+        L.Sur_Get_Grads.argtypes = [fp]
+        L.Sur_Set_Scale.argtypes = [fp]
+        L.Sur_Forward.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.Sur_Gradients.argtypes = [vp, vp, vp, ctypes.c_int, dp]
+        L.Sur_Train_Step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float, dp]
+        L.Sur_Time_Forward.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.Sur_Time_Forward.restype = ctypes.c_float
+        self.n = L.Sur_Num_Params()
+        assert self.n == num_params()
+        self.max_batch = max_batch
+        if L.Sur_Init(H, W, max_batch) != 0:
+            raise RuntimeError("Sur_Init failed (already initialised in this process?)")
+
+    @classmethod
+    def get(cls, max_batch: int = 16) -> "Surrogate":
+        if cls._instance is None:
+            cls._instance = Surrogate(max_batch)
+        if cls._instance.max_batch < max_batch:
+            raise ValueError("the surrogate library was initialised with a smaller batch")
+        return cls._instance
+
+    def buffer(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self.lib, nbytes)
+
+    def set_params(self, theta: np.ndarray) -> None:
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == self.n
+        self.lib.Sur_Set_Params(t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+    def get_params(self) -> np.ndarray:
+        t = np.empty(self.n, dtype=np.float32)
+        self.lib.Sur_Get_Params(t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return t
+
+    def get_grads(self) -> np.ndarray:
+        t = np.empty(self.n, dtype=np.float32)
+        self.lib.Sur_Get_Grads(t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return t
+
+    def set_scale(self, s0: float, s1: float) -> None:
+        s = np.array([s0, s1], dtype=np.float32)
+        self.lib.Sur_Set_Scale(s.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+    def forward(self, state: DeviceBuffer, fk: DeviceBuffer, out: DeviceBuffer, B: int) -> None:
+        if self.lib.Sur_Forward(state.ptr, fk.ptr, out.ptr, B) != 0:
+            raise ValueError("bad forward call")
+
+    def gradients(self, state, fk, target, B: int) -> float:
+        loss = ctypes.c_double()
+        if self.lib.Sur_Gradients(state.ptr, fk.ptr, target.ptr, B, ctypes.byref(loss)) != 0:
+            raise ValueError("bad gradient call")
+        return loss.value
+
+    def train_step(self, state, fk, target, B: int, lr: float) -> float:
+        loss = ctypes.c_double()
+        if self.lib.Sur_Train_Step(state.ptr, fk.ptr, target.ptr, B, ctypes.c_float(lr), ctypes.byref(loss)) != 0:
+            raise ValueError("bad train call")
+        return loss.value
+
+    def time_forward(self, state, fk, out, B: int, reps: int) -> float:
+        return float(self.lib.Sur_Time_Forward(state.ptr, fk.ptr, out.ptr, B, reps))
+
+    def gather(self, ptrs: Sequence[int], dst: DeviceBuffer, n_each: int) -> None:
+        arr = (ctypes.c_ulonglong * len(ptrs))(*ptrs)
+        self.lib.Sur_Gather(arr, ctypes.c_void_p(dst.ptr), len(ptrs), ctypes.c_size_t(n_each))
+
+
+# --------------------------------------------------------------------------
+def trajectory_specs(n: int, seed: int) -> List[InstanceSpec]:
+    """n instances of 256^2 with F, k uniform in the surrogate's ranges."""
+    from .rng import uniform
+    u = uniform(seed, (n, 2))
+    return [InstanceSpec(H, W, F_RANGE[0] + (F_RANGE[1] - F_RANGE[0]) * u[i, 0],
+                         K_RANGE[0] + (K_RANGE[1] - K_RANGE[0]) * u[i, 1]) for i in range(n)]
+
+
+class Stencil:
+    """The FP32 stencil for the surrogate's instances: the batched library
+    (gs2d_batch_f32) holding one instance per trajectory."""
+
+    def __init__(self, specs: Sequence[InstanceSpec], plan=(2, 2, 32), name: str = "gs2d_batch_f32"):
+        self.lib = open_batch(name)
+        self.lib.init(specs)
+        self.specs = list(specs)
+        self.lib.plan([0] * len(specs), [plan])
+
+    def load(self, sur: Surrogate, src: DeviceBuffer, first: int = 0, count: Optional[int] = None) -> None:
+        """Copy states [count][2][H][W] from a device buffer into instances first..first+count-1."""
+        count = len(self.specs) - first if count is None else count
+        nb = H * W * 4
+        for j in range(count):
+            for f in range(2):
+                dst = self.lib.lib.Formura_Batch_Device_Field(first + j, f)
+                sur.lib.Sur_Copy(ctypes.c_void_p(dst), ctypes.c_void_p(src.ptr + (2 * j + f) * nb), nb, 2)
+
+    def store(self, sur: Surrogate, dst: DeviceBuffer, first: int = 0, count: Optional[int] = None) -> None:
+        count = len(self.specs) - first if count is None else count
+        ptrs = []
+        for j in range(count):
+            for f in range(2):
+                ptrs.append(self.lib.lib.Formura_Batch_Device_Field(first + j, f))
+        sur.gather(ptrs, dst, H * W)
+
+    def advance(self, steps: int) -> float:
+        return self.lib.advance(steps, graph=True)
+
+
+def fk_array(specs: Sequence[InstanceSpec]) -> np.ndarray:
+    return np.array([[s.F, s.k] for s in specs], dtype=np.float32)
+
+
+def initial_states(specs: Sequence[InstanceSpec], seed: int) -> np.ndarray:
+    from .rng import gray_scott_ic
+    out = np.empty((len(specs), 2, H, W), dtype=np.float32)
+    for i, s in enumerate(specs):
+        u, v = gray_scott_ic((H, W), seed + i)
+        out[i, 0], out[i, 1] = u, v
+    return out
+
+
+@dataclass
+class TrainResult:
+    losses: List[float] = field(default_factory=list)
+    val_errors: List[float] = field(default_factory=list)
+    seconds: float = 0.0
+    steps: int = 0
+    scale: tuple = (1.0, 1.0)
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
+          seed: int = 7, log=print) -> TrainResult:
+    """Online training on trajectories of the FP32 stencil.
+
+    All n_traj instances advance together through the batched stencil; after
+    each 32-step advance the (state, state + 32 steps) pairs of that round are
+    the training set for the next steps of Adam (minibatches of `batch`, each
+    pair used once per round).  Stops after traj_steps stencil steps or
+    max_seconds."""
+    sur = Surrogate.get(batch)
+    specs = trajectory_specs(n_traj, seed)
+    st = Stencil(specs)
+    n_each = 2 * H * W
+    x0 = initial_states(specs, seed * 1000)
+    cur = sur.buffer(4 * n_each * n_traj)
+    nxt = sur.buffer(4 * n_each * n_traj)
+    fk = sur.buffer(4 * 2 * n_traj)
+    fk.upload(fk_array(specs))
+    for i in range(n_traj):
+        st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
+    theta = init_params(seed)
+    sur.set_params(theta)
+    res = TrainResult()
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    # burn-in of 64 steps so the first pairs are not the raw initial condition
+    st.advance(64)
+    st.store(sur, cur)
+    rounds = (traj_steps - 64) // SUPER
+    bstate = sur.buffer(4 * n_each * batch)
+    btarget = sur.buffer(4 * n_each * batch)
+    bfk = sur.buffer(4 * 2 * batch)
+    fk_host = fk_array(specs)
+    for r in range(rounds):
+        st.advance(SUPER)
+        st.store(sur, nxt)
+        if r == 0:
+            a = cur.download((n_traj, 2, H, W))
+            b = nxt.download((n_traj, 2, H, W))
+            inc = (b - a).astype(np.float64)
+            scale = (float(inc[:, 0].std()), float(inc[:, 1].std()))
+            sur.set_scale(*scale)
+            res.scale = scale
+            log(f"output scale from the first round: {scale}")
+        order = rng.permutation(n_traj)
+        for j in range(0, n_traj - batch + 1, batch):
+            idx = order[j:j + batch]
+            for q, i in enumerate(idx):
+                nb = 4 * n_each
+                sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(cur.ptr + int(i) * nb), nb, 2)
+                sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(nxt.ptr + int(i) * nb), nb, 2)
+            bfk.upload(fk_host[idx])
+            step_lr = lr * (0.5 * (1 + math.cos(math.pi * min(1.0, (time.perf_counter() - t0) / max_seconds))))
+            loss = sur.train_step(bstate, bfk, btarget, batch, step_lr)
+            res.losses.append(loss)
+            res.steps += 1
+        cur, nxt = nxt, cur
+        el = time.perf_counter() - t0
+        if r % 8 == 0:
+            log(f"round {r}/{rounds}: {res.steps} Adam steps, loss {np.mean(res.losses[-50:]):.3e}, {el:.1f} s")
+        if el > max_seconds:
+            break
+    res.seconds = time.perf_counter() - t0
+    for buf in (cur, nxt, fk, bstate, btarget, bfk):
+        buf.free()
+    return res

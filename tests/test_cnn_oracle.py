@@ -1,0 +1,62 @@
+"""Pins for the CNN oracle: its float64 gradients against central finite
+differences of its own loss, on a small image; BF16 rounding is
+round-to-nearest-even; the circular convolution and its adjoint satisfy
+<conv(a), z> = <a, conv_t(z)>."""
+from __future__ import annotations
+
+import numpy as np
+
+from formura_amd.surrogate import init_params
+from oracle import cnn_oracle as O
+
+
+def small_case(seed=0, Hs=6, Ws=5):
+    rng = np.random.default_rng(seed)
+    state = rng.uniform(0, 1, (1, 2, Hs, Ws))
+    target = state + rng.normal(0, 0.05, state.shape)
+    fk = np.array([[0.03, 0.06]])
+    theta = init_params(seed).astype(np.float64)
+    theta[-(2 * 9 * 48 + 2):] *= 10.0        # a last layer large enough to matter
+    return theta, state, target, fk
+
+
+def test_gradients_match_finite_differences():
+    theta, state, target, fk = small_case()
+    scale = (0.3, 0.2)
+    _, g = O.loss_and_grad(theta, state, fk, target, scale, quantize=False)
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([rng.choice(theta.size, 12, replace=False),
+                          [0, 4 * 9 * 4 + 3, theta.size - 1, theta.size - 3]])
+    h = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        lp, _ = O.loss_and_grad(tp, state, fk, target, scale, quantize=False)
+        lm, _ = O.loss_and_grad(tm, state, fk, target, scale, quantize=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1e-8, abs(fd)) + 1e-12, (i, fd, g[i])
+
+
+def test_conv_adjoint_identity():
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=(2, 5, 7, 3))
+    z = rng.normal(size=(2, 5, 7, 4))
+    w = rng.normal(size=(4, 3, 3, 3))
+    lhs = float((O.conv(a, w) * z).sum())
+    rhs = float((a * O.conv_t(z, w)).sum())
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # the weight gradient is the derivative of <conv(a, w), z> in w
+    gw = O.conv_w(a, z)
+    dw = rng.normal(size=w.shape)
+    assert abs(float((O.conv(a, dw) * z).sum()) - float((gw * dw).sum())) <= 1e-10 * abs(lhs)
+
+
+def test_bf16_round_to_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e-30, 65504.0], dtype=np.float32)
+    r = O.bf16(x)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: even -> 1; 1 + 3*2^-8 rounds up to 1 + 2^-6
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6
+    assert r[3] == -2.5
+    assert r[5] == 65536.0
+    assert abs(r[4] - 3.0e-30) / 3.0e-30 <= 2 ** -8

@@ -1,0 +1,86 @@
+"""Surrogate CNN kernels against the CPU oracle (oracle/cnn_oracle.py): the
+forward pass, the loss and every gradient, on the same BF16-rounded weights
+and inputs.  Tolerances: forward state 1e-5 relative L2 (the residual output
+is dominated by the FP32 input state), loss 1e-3 relative, each layer's weight
+and bias gradient 2e-2 relative L2 (FP32 accumulation orders differ and
+BF16 roundings of intermediates may land one unit apart)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _case(B=2, seed=3):
+    from formura_amd import surrogate as S
+    specs = S.trajectory_specs(B, seed)
+    state = S.initial_states(specs, seed)
+    rng = np.random.default_rng(seed)
+    state = state + rng.normal(0, 0.05, state.shape).astype(np.float32)
+    target = state + rng.normal(0, 0.01, state.shape).astype(np.float32)
+    return specs, state, target, S.fk_array(specs)
+
+
+def test_forward_and_gradients_match_oracle():
+    from formura_amd import surrogate as S
+    from oracle import cnn_oracle as O
+    sur = S.Surrogate.get(16)
+    specs, state, target, fk = _case()
+    B = len(specs)
+    theta = S.init_params(11)
+    # make the last layer non-trivial so every gradient path is exercised
+    theta = theta * np.float32(1.0)
+    sur.set_params(theta)
+    scale = (0.02, 0.01)
+    sur.set_scale(*scale)
+    bs, bt, bf, bo = (sur.buffer(state.nbytes), sur.buffer(target.nbytes), sur.buffer(fk.nbytes),
+                      sur.buffer(state.nbytes))
+    bs.upload(state)
+    bt.upload(target)
+    bf.upload(fk)
+    _say("[surrogate] forward")
+    sur.forward(bs, bf, bo, B)
+    out = bo.download(state.shape)
+    ref = O.forward(theta.astype(np.float64), state.astype(np.float64), fk.astype(np.float64), scale)
+    inc_gpu = out.astype(np.float64) - state
+    inc_ref = ref["out"] - state
+    e_inc = np.linalg.norm(inc_gpu - inc_ref) / np.linalg.norm(inc_ref)
+    _say(f"[surrogate] forward increment rel L2 error {e_inc:.3e}")
+    assert e_inc < 2e-2
+    assert np.linalg.norm(out - ref["out"]) / np.linalg.norm(ref["out"]) < 1e-5
+    _say("[surrogate] gradients")
+    loss = sur.gradients(bs, bf, bt, B)
+    g = sur.get_grads().astype(np.float64)
+    lref, gref = O.loss_and_grad(theta.astype(np.float64), state.astype(np.float64), fk.astype(np.float64),
+                                 target.astype(np.float64), scale)
+    _say(f"[surrogate] loss GPU {loss:.6e} oracle {lref:.6e}")
+    assert abs(loss - lref) <= 1e-3 * lref
+    off = 0
+    for l, (ci, co) in enumerate(zip(O.CIN, O.COUT)):
+        nw, nb = co * 9 * ci, co
+        for name, sl in (("W", slice(off, off + nw)), ("b", slice(off + nw, off + nw + nb))):
+            e = np.linalg.norm(g[sl] - gref[sl]) / max(np.linalg.norm(gref[sl]), 1e-30)
+            _say(f"[surrogate] layer {l + 1} d{name}: rel L2 error {e:.3e}")
+            assert e < 2e-2, f"layer {l + 1} d{name}"
+        off += nw + nb
+    for b in (bs, bt, bf, bo):
+        b.free()
+
+
+def test_training_reduces_loss():
+    from formura_amd import surrogate as S
+    _say("[surrogate] short training run")
+    res = S.train(n_traj=64, traj_steps=64 + 32 * 6, batch=16, lr=2e-3, max_seconds=60.0,
+                  log=lambda m: _say("[surrogate] " + m))
+    first = float(np.mean(res.losses[:4]))
+    last = float(np.mean(res.losses[-4:]))
+    _say(f"[surrogate] loss {first:.3e} -> {last:.3e} over {res.steps} Adam steps")
+    assert last < 0.5 * first
