@@ -288,3 +288,182 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     for buf in (cur, nxt, fk, bstate, btarget, bfk):
         buf.free()
     return res
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gs2d_256.npz")
+
+
+def save_weights(path: str, theta: np.ndarray, scale, meta: Dict) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, theta=theta.astype(np.float32), scale=np.array(scale, dtype=np.float64),
+             meta=np.array(repr(meta)))
+
+
+def load_weights(path: str = WEIGHTS):
+    with np.load(path, allow_pickle=False) as z:
+        return z["theta"].astype(np.float32), tuple(float(x) for x in z["scale"]), str(z["meta"])
+
+
+@dataclass
+class Rollout:
+    state: np.ndarray
+    cnn_steps: int = 0
+    stencil_supersteps: int = 0
+    checks: List[float] = field(default_factory=list)
+    fallback_at: Optional[int] = None
+    seconds: float = 0.0
+
+
+def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarray, n_super: int,
+                    check_every: int = CHECK_EVERY, bound: float = BOUND) -> Rollout:
+    """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each)
+    with the CNN; every `check_every`-th super-step is also computed by the
+    FP32 stencil from the same state, its result is kept, and if the CNN's
+    relative L2 difference on any trajectory exceeds `bound` the stencil
+    advances the rest of the rollout."""
+    B = x0.shape[0]
+    nbytes = x0.nbytes
+    a, b, ref = sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes)
+    a.upload(x0)
+    res = Rollout(state=None)
+    t0 = time.perf_counter()
+    for j in range(n_super):
+        if res.fallback_at is not None:
+            st.load(sur, a, 0, B)
+            st.advance(SUPER)
+            st.store(sur, a, 0, B)
+            res.stencil_supersteps += 1
+            continue
+        sur.forward(a, fk, b, B)
+        res.cnn_steps += 1
+        if (j + 1) % check_every == 0:
+            st.load(sur, a, 0, B)
+            st.advance(SUPER)
+            st.store(sur, ref, 0, B)
+            res.stencil_supersteps += 1
+            p = b.download(x0.shape)
+            q = ref.download(x0.shape)
+            err = max(rel_l2(p[i], q[i]) for i in range(B))
+            res.checks.append(err)
+            a, ref = ref, a                  # keep the stencil's state
+            if err > bound:
+                res.fallback_at = j
+        else:
+            a, b = b, a
+    res.state = a.download(x0.shape)
+    res.seconds = time.perf_counter() - t0
+    for buf in (a, b, ref):
+        buf.free()
+    return res
+
+
+def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed: int = 99,
+             bound: float = BOUND, log=print) -> Dict:
+    """Checked surrogate vs the stencil on held-out trajectories.
+
+    Reference: the FP64 batch library (bit-equal to the CPU oracle) run for
+    the same 32*n_super steps.  Timings: the checked surrogate; the plain FP32
+    and FP64 stencils (one single-site-per-thread launch per instance per
+    step); the batched FP32 stencil (fused, hipGraph)."""
+    sur = Surrogate.get(max(16, n_traj))
+    sur.set_params(theta)
+    sur.set_scale(*scale)
+    specs = trajectory_specs(n_traj, seed)
+    x0 = initial_states(specs, seed * 1000)
+    steps = SUPER * n_super
+    fk = sur.buffer(8 * n_traj)
+    fk.upload(fk_array(specs))
+    # burn-in with the FP32 stencil so the rollout starts from a developed pattern
+    st = Stencil(specs)
+    for i in range(n_traj):
+        st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
+    st.advance(256)
+    start = sur.buffer(x0.nbytes)
+    st.store(sur, start)
+    xs = start.download(x0.shape)
+    out: Dict = {"trajectories": n_traj, "super_steps": n_super, "steps": steps, "bound": bound}
+    # FP64 reference
+    ref64 = open_batch("gs2d_batch_f64")
+    ref64.init(specs)
+    ref64.plan([0] * n_traj, [(2, 2, 32)])
+    for i in range(n_traj):
+        ref64.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+    ref64.advance(steps, graph=False)
+    truth = np.stack([np.stack(ref64.get_state(i)) for i in range(n_traj)])
+    # checked surrogate (warm-up first: one forward and one stencil super-step)
+    st.load(sur, start)
+    st.advance(SUPER)
+    ro = checked_rollout(sur, st, fk, xs, n_super, bound=bound)
+    out["surrogate"] = {"seconds": round(ro.seconds, 4), "cnn_supersteps": ro.cnn_steps,
+                        "stencil_supersteps": ro.stencil_supersteps, "fallback_at": ro.fallback_at,
+                        "check_errors": [float(f"{e:.3e}") for e in ro.checks],
+                        "rel_l2_vs_fp64": rel_l2(ro.state.astype(np.float64), truth)}
+    log(f"checked surrogate: {out['surrogate']}")
+    # plain stencils and the batched stencil from the same state
+    for name, key in (("gs2d_batch_f32", "plain_fp32"), ("gs2d_batch_f64", "plain_fp64")):
+        lib = open_batch(name)
+        if lib.specs != specs:
+            lib.init(specs)
+            lib.plan([0] * n_traj, [(2, 2, 32)])
+        for i in range(n_traj):
+            lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+        lib.advance_plain(2)
+        for i in range(n_traj):
+            lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+        t0 = time.perf_counter()
+        lib.advance_plain(steps)
+        sec = time.perf_counter() - t0
+        res = np.stack([np.stack(lib.get_state(i)) for i in range(n_traj)])
+        out[key] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth)}
+        log(f"{key}: {out[key]}")
+    st.lib.plan([0] * n_traj, [(8, 2, 32)])
+    for i in range(n_traj):
+        st.lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+    st.advance(steps)   # capture
+    for i in range(n_traj):
+        st.lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+    t0 = time.perf_counter()
+    st.advance(steps)
+    sec = time.perf_counter() - t0
+    res = np.stack([np.stack(st.lib.get_state(i)) for i in range(n_traj)])
+    out["batched_fp32"] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth)}
+    log(f"batched_fp32: {out['batched_fp32']}")
+    s = out["surrogate"]["seconds"]
+    out["speedup_vs_plain_fp32"] = round(out["plain_fp32"]["seconds"] / s, 2)
+    out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["seconds"] / s, 2)
+    out["speedup_vs_batched_fp32"] = round(out["batched_fp32"]["seconds"] / s, 3)
+    for b in (fk, start):
+        b.free()
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m formura_amd.surrogate")
+    ap.add_argument("command", choices=["train", "evaluate"])
+    ap.add_argument("--seconds", type=float, default=240.0)
+    ap.add_argument("--trajectories", type=int, default=2000)
+    ap.add_argument("--traj-steps", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--weights", default=WEIGHTS)
+    a = ap.parse_args(argv)
+    if a.command == "train":
+        res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, log=lambda m: print(m, flush=True))
+        theta = Surrogate.get(a.batch).get_params()
+        meta = {"trajectories": a.trajectories, "traj_steps": a.traj_steps, "adam_steps": res.steps,
+                "seconds": round(res.seconds, 1), "final_loss": float(np.mean(res.losses[-100:]))}
+        save_weights(a.out, theta, res.scale, meta)
+        print(json.dumps(meta), flush=True)
+        ev = evaluate(theta, res.scale, log=lambda m: print(m, flush=True))
+        print(json.dumps(ev), flush=True)
+    else:
+        theta, scale, meta = load_weights(a.weights)
+        print(meta, flush=True)
+        print(json.dumps(evaluate(theta, scale, log=lambda m: print(m, flush=True))), flush=True)
+
+
+if __name__ == "__main__":
+    main()
