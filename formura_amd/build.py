@@ -96,6 +96,25 @@ def build_copy_probe() -> str:
     return out
 
 
+def build_driver(name: str = "gs3d64", oracle_lib: Optional[str] = None) -> str:
+    """examples/gray-scott-3d-main.cpp linked against the HIP library of a 3D
+    Gray-Scott program (or, with oracle_lib, against that CPU library)."""
+    if oracle_lib:
+        libdir, lib, out = os.path.dirname(oracle_lib), os.path.basename(oracle_lib)[3:-3], oracle_lib[:-3] + "_main"
+        extra = ["-fopenmp"]
+        incdir = libdir
+    else:
+        libdir, lib, out = BUILD, name, os.path.join(BUILD, f"{name}_main")
+        extra = []
+        incdir = BUILD
+    cmd = ["g++", "-O2", f'-DGS3D_HEADER="{name}.h"', "-I", incdir, "-I", MPI_STUB,
+           os.path.join(EXAMPLES, "gray-scott-3d-main.cpp"), "-L", libdir, f"-l{lib}",
+           f"-Wl,-rpath,{libdir}"] + extra + ["-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(names=None, jobs: int = 8) -> Dict[str, str]:
     os.makedirs(BUILD, exist_ok=True)
     if names is None:
@@ -108,6 +127,8 @@ def build_all(names=None, jobs: int = 8) -> Dict[str, str]:
         futs = {ex.submit(compile_hip, os.path.join(BUILD, f"{n}.hip"), lib_path(n)): n for n in names}
         for f in cf.as_completed(futs):
             out[futs[f]] = f.result()
+    if "gs3d64" in names:
+        out["gs3d64_main"] = build_driver("gs3d64")
     return out
 
 

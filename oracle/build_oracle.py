@@ -75,7 +75,11 @@ def build_oracle(name: str) -> str:
 
 def build_all(names=None):
     names = names or (list(product_build.PROGRAMS) + list(ORACLE_ONLY))
-    return {n: build_oracle(n) for n in names}
+    out = {n: build_oracle(n) for n in names}
+    if "gs3d64" in out:
+        # the example driver, linked against the CPU library of the same program
+        out["gs3d64_main"] = product_build.build_driver("gs3d64", oracle_lib=out["gs3d64"])
+    return out
 
 
 if __name__ == "__main__":
