@@ -48,7 +48,7 @@ def seeded_state(lib, seed):
 
 def candidates(lib):
     from formura_amd.runtime import KIND_DIRECT, KIND_MARCH
-    out = [((KIND_DIRECT, 1, 0, 1, 1), 64)]
+    out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)]
     for v in lib.list_kernels():
         for lx in (32, 64, 128):
             out.append(((KIND_MARCH,) + tuple(v), lx))
@@ -120,7 +120,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,lx: skip the search")
+    ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,occ,lx: skip the search")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -138,7 +138,7 @@ def main():
     sweep = {}
     if a.variant:
         v = [int(x) for x in a.variant.split(",")]
-        var, lx = tuple(v[:5]), v[5]
+        var, lx = tuple(v[:6]), v[6]
     else:
         log("autotuning heat512 FP64 ...")
         best = autotune(lib)
@@ -177,7 +177,8 @@ def main():
         "data": "synthetic: u ~ N(0,1) i.i.d. (numpy PCG64 seed 1)",
         "config": {"workload": "heat3d 7-point, 512^3 periodic, FP64, D=1 dx=1 dt=0.1",
                    "grid": [512, 512, 512], "kernel": {"kind": var[0], "T": var[1], "rows_per_wave": var[2],
-                                                       "waves": var[3], "prefetch": var[4], "march_chunk": lx}},
+                                                       "waves": var[3], "prefetch": var[4], "waves_per_simd": var[5],
+                                                       "march_chunk": lx}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "t_sweep_glups": sweep,

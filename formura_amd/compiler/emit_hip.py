@@ -28,6 +28,7 @@ from .pointir import PointIR, c_expr_of, imm_literal
 # spill to scratch memory.
 MARCH_REG_BUDGET = 340
 MAX_WAVES = 4        # <= 4 waves per workgroup keeps one wave per SIMD (512 registers)
+OCC2_REG_BUDGET = 200   # two waves per SIMD: 256 registers per wave
 
 
 def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
@@ -55,15 +56,22 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
         W = -(-rows // by_max)
         by = -(-rows // W)
         if W > 1:
-            out.append((T, by, W, 1))
+            out.append((T, by, W, 1, 1))
         by1 = min(by_max, 64 + 2 * T)
         if by1 >= 2 * T + 2:
-            out.append((T, by1, 1, 1))
+            out.append((T, by1, 1, 1, 1))
         by2 = 1
         while march_regs(T, by2 + 1, nf, words, 2) <= MARCH_REG_BUDGET:
             by2 += 1
         if by2 >= 2 * T + 2:
-            out.append((T, min(by2, 64 + 2 * T), 1, 2))
+            out.append((T, min(by2, 64 + 2 * T), 1, 2, 1))
+        # two waves per SIMD: half the registers per wave, twice the waves
+        # to hide memory and arithmetic latency
+        by3 = 1
+        while march_regs(T, by3 + 1, nf, words, 2) <= OCC2_REG_BUDGET:
+            by3 += 1
+        if by3 >= 2 * T + 4:
+            out.append((T, by3, 1, 2, 2))
     return out
 
 
@@ -140,10 +148,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("};")
     w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
-    for T, by, wv, pf in variants:
-        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}}}, &fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}>}},")
+    for T, by, wv, pf, occ in variants:
+        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}, {occ}}}, "
+          f"&fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}, {occ}>}},")
     if not variants:
-        w("  {{fmr::KIND_DIRECT, 1, 0, 1, 1}, nullptr},")
+        w("  {{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}, nullptr},")
     w("};")
     nv = len(variants)
     w(f"constexpr int kNumVariants = {nv};")
@@ -208,11 +217,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Set_Kernel (int kind, int T, int rows, int waves, int pf)")
+    w("int Formura_Set_Kernel (int kind, int T, int rows, int waves, int pf, int occ)")
     w("{")
-    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1}; return 0; }")
-    w("  if (!lib.find(kind, T, rows, waves, pf)) return -1;")
-    w("  lib.choice = fmr::Variant{kind, T, rows, waves, pf};")
+    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}; return 0; }")
+    w("  if (!lib.find(kind, T, rows, waves, pf, occ)) return -1;")
+    w("  lib.choice = fmr::Variant{kind, T, rows, waves, pf, occ};")
     w("  return 0;")
     w("}")
     w("")
@@ -223,10 +232,10 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
-    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *waves, int *pf, int *lx)")
+    w("int Formura_Get_Kernel (int *kind, int *T, int *rows, int *waves, int *pf, int *occ, int *lx)")
     w("{")
     w("  *kind = lib.choice.kind; *T = lib.choice.T; *rows = lib.choice.by; *waves = lib.choice.waves;")
-    w("  *pf = lib.choice.pf;")
+    w("  *pf = lib.choice.pf; *occ = lib.choice.occ;")
     w("  *lx = lib.lx;")
     w("  return 0;")
     w("}")
@@ -235,8 +244,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("{")
     w("  int k = 0;")
     w("  for (int i = 0; i < kNumVariants && k < max_variants; ++i, ++k) {")
-    w("    out[4 * k] = kVariants[i].v.T; out[4 * k + 1] = kVariants[i].v.by; out[4 * k + 2] = kVariants[i].v.waves;")
-    w("    out[4 * k + 3] = kVariants[i].v.pf;")
+    w("    out[5 * k] = kVariants[i].v.T; out[5 * k + 1] = kVariants[i].v.by; out[5 * k + 2] = kVariants[i].v.waves;")
+    w("    out[5 * k + 3] = kVariants[i].v.pf; out[5 * k + 4] = kVariants[i].v.occ;")
     w("  }")
     w("  return k;")
     w("}")

@@ -80,18 +80,19 @@ struct Variant {
   int by;      // rows per lane (march kernels)
   int waves;   // wavefronts per tile (march kernels)
   int pf;      // prefetch depth in planes (march kernels)
+  int occ;     // minimum waves per SIMD requested from the compiler (march kernels)
 };
 
 template <class P>
 using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
                                int, Params, hipStream_t);
 
-template <class P, int T, int BY, int W, int PF>
+template <class P, int T, int BY, int W, int PF, int OCC>
 void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int lx,
                     int nty, int ntz, int ntx, Params prm, hipStream_t s) {
   const int ntot = nty * ntz * ntx;
   const int per = (ntot + N_XCD - 1) / N_XCD;
-  hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF>), dim3(per * N_XCD), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
+  hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(per * N_XCD), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
                      lx, nty, ntz, ntx, prm);
 }
 
@@ -114,7 +115,7 @@ class StepLibrary {
   int nparams = 0;
   const VariantEntry<P>* variants = nullptr;
   int nvariants = 0;
-  Variant choice{KIND_DIRECT, 1, 0, 1, 1};
+  Variant choice{KIND_DIRECT, 1, 0, 1, 1, 1};
   int lx = 64;
   double* red_part = nullptr;
   double* red_res = nullptr;
@@ -142,7 +143,7 @@ class StepLibrary {
     FMR_CHECK(hipEventCreate(&ev0));
     FMR_CHECK(hipEventCreate(&ev1));
     // default: the fused variant with T = 2 if compiled in, else the first one
-    choice = Variant{KIND_DIRECT, 1, 0, 1, 1};
+    choice = Variant{KIND_DIRECT, 1, 0, 1, 1, 1};
     for (int i = 0; i < nvariants; ++i)
       if (choice.kind == KIND_DIRECT || (variants[i].v.T == 2 && choice.T != 2)) choice = variants[i].v;
     FMR_CHECK(hipStreamSynchronize(stream));
@@ -185,10 +186,10 @@ class StepLibrary {
       }
   }
 
-  const VariantEntry<P>* find(int kind, int T, int by, int waves, int pf) const {
+  const VariantEntry<P>* find(int kind, int T, int by, int waves, int pf, int occ) const {
     for (int i = 0; i < nvariants; ++i)
       if (variants[i].v.kind == kind && variants[i].v.T == T && variants[i].v.by == by &&
-          variants[i].v.waves == waves && variants[i].v.pf == pf)
+          variants[i].v.waves == waves && variants[i].v.pf == pf && variants[i].v.occ == occ)
         return &variants[i];
     return nullptr;
   }
@@ -224,10 +225,10 @@ class StepLibrary {
     int left = steps;
     while (left > 0) {
       const VariantEntry<P>* e = nullptr;
-      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf);
+      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
       if (!e) {   // remainder: deepest variant with the same tile shape that fits
         for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t)
-          e = find(KIND_MARCH, t, choice.by, choice.waves, choice.pf);
+          e = find(KIND_MARCH, t, choice.by, choice.waves, choice.pf, choice.occ);
         for (int i = 0; i < nvariants && !e; ++i)
           if (variants[i].v.T <= left) e = &variants[i];
       }

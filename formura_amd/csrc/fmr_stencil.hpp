@@ -153,6 +153,9 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // load; results are written with non-temporal stores so they do not evict
 // the ghost rows the neighbouring tiles are about to read from L2.
 //
+// OCC asks the compiler for at least OCC waves per SIMD (smaller register
+// budget per wave, more waves to hide latency).
+//
 // Workgroup -> tile mapping is XCD-aware: consecutive workgroups go to
 // different XCDs, so tile t is given to workgroup (t % per)*8 + t / per, which
 // keeps neighbouring tiles -- whose ghost rows overlap -- on one XCD and its L2.
@@ -290,8 +293,8 @@ struct March3D {
   }
 };
 
-template <class P, int T, int BY, int W, int PF>
-__global__ void __launch_bounds__(64 * W) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
+template <class P, int T, int BY, int W, int PF, int OCC>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC))) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
                                                          int nx, int ny, int nz, int lx, int nty, int ntz, int ntx,
                                                          Params prm) {
   using M = March3D<P, T, BY, W, PF>;

@@ -299,6 +299,20 @@ __global__ void sum_chunks(const float* __restrict__ part, float* __restrict__ o
   }
 }
 
+// x += sigma * N(0,1) with a counter-based hash (Box-Muller over two uniforms):
+// input-noise augmentation for training
+__global__ void add_noise(float* __restrict__ x, size_t n, float sigma, unsigned long long seed) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ULL * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    const float u1 = ((float)(z >> 40) + 0.5f) * (1.0f / 16777216.0f);
+    const float u2 = ((float)((z >> 16) & 0xffffffULL)) * (1.0f / 16777216.0f);
+    x[i] += sigma * sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+  }
+}
+
 // Adam on the FP32 master parameters
 __global__ void adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
                      float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -593,6 +607,13 @@ int Sur_Gather(const unsigned long long* host_ptrs, float* dst, int nptr, size_t
   for (int i = 0; i < nptr; ++i)
     SCHECK(hipMemcpyAsync(dst + (size_t)i * n_each, (const void*)host_ptrs[i], n_each * sizeof(float),
                           hipMemcpyDeviceToDevice, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
+int Sur_Add_Noise(float* x, size_t n, float sigma, unsigned long long seed) {
+  hipLaunchKernelGGL(add_noise, dim3(grid_for(n, 256)), dim3(256), 0, net.stream, x, n, sigma, seed);
+  SCHECK(hipGetLastError());
   SCHECK(hipStreamSynchronize(net.stream));
   return 0;
 }

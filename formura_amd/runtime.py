@@ -59,9 +59,9 @@ class StepLibrary:
             L.Formura_Forward_Device.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
             L.Formura_Forward_Device_Timed.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
             L.Formura_Forward_Device_Timed.restype = ctypes.c_float
-            L.Formura_Set_Kernel.argtypes = [ctypes.c_int] * 5
+            L.Formura_Set_Kernel.argtypes = [ctypes.c_int] * 6
             L.Formura_Set_March_Chunk.argtypes = [ctypes.c_int]
-            L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 6
+            L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 7
             L.Formura_List_Kernels.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
             L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.Formura_Device_Field.argtypes = [ctypes.c_int]
@@ -109,25 +109,26 @@ class StepLibrary:
     def forward_device_timed(self, nsteps: int) -> float:
         return float(self.lib.Formura_Forward_Device_Timed(ctypes.byref(self.navi), int(nsteps)))
 
-    def set_kernel(self, kind: int, T: int = 1, rows: int = 0, waves: int = 1, pf: int = 1) -> None:
-        if self.lib.Formura_Set_Kernel(kind, T, rows, waves, pf) != 0:
-            raise ValueError(f"kernel variant kind={kind} T={T} rows={rows} waves={waves} pf={pf} is not compiled in")
+    def set_kernel(self, kind: int, T: int = 1, rows: int = 0, waves: int = 1, pf: int = 1, occ: int = 1) -> None:
+        if self.lib.Formura_Set_Kernel(kind, T, rows, waves, pf, occ) != 0:
+            raise ValueError(f"kernel variant kind={kind} T={T} rows={rows} waves={waves} pf={pf} occ={occ} "
+                             "is not compiled in")
 
     def set_chunk(self, lx: int) -> None:
         if self.lib.Formura_Set_March_Chunk(int(lx)) != 0:
             raise ValueError("bad march chunk")
 
     def get_kernel(self):
-        """(kind, T, rows per wave, waves, prefetch depth, march chunk)"""
-        v = [ctypes.c_int() for _ in range(6)]
+        """(kind, T, rows per wave, waves, prefetch depth, waves per SIMD, march chunk)"""
+        v = [ctypes.c_int() for _ in range(7)]
         self.lib.Formura_Get_Kernel(*[ctypes.byref(x) for x in v])
         return tuple(x.value for x in v)
 
     def list_kernels(self) -> List[tuple]:
-        """Compiled march variants as (T, rows per wave, waves, prefetch depth)."""
-        buf = (ctypes.c_int * 256)()
+        """Compiled march variants as (T, rows per wave, waves, prefetch depth, waves per SIMD)."""
+        buf = (ctypes.c_int * 320)()
         n = self.lib.Formura_List_Kernels(buf, 64)
-        return [tuple(buf[4 * i + j] for j in range(4)) for i in range(n)]
+        return [tuple(buf[5 * i + j] for j in range(5)) for i in range(n)]
 
     def set_params(self, p: Sequence[float]) -> None:
         arr = (ctypes.c_double * len(p))(*p)

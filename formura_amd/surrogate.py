@@ -96,6 +96,7 @@ class Surrogate:
         L.Sur_Train_Step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float, dp]
         L.Sur_Time_Forward.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.Sur_Time_Forward.restype = ctypes.c_float
+        L.Sur_Add_Noise.argtypes = [vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_ulonglong]
         self.n = L.Sur_Num_Params()
         assert self.n == num_params()
         self.max_batch = max_batch
@@ -150,6 +151,10 @@ class Surrogate:
 
     def time_forward(self, state, fk, out, B: int, reps: int) -> float:
         return float(self.lib.Sur_Time_Forward(state.ptr, fk.ptr, out.ptr, B, reps))
+
+    def add_noise(self, buf: DeviceBuffer, n: int, sigma: float, seed: int) -> None:
+        self.lib.Sur_Add_Noise(ctypes.c_void_p(buf.ptr), ctypes.c_size_t(n), ctypes.c_float(sigma),
+                               ctypes.c_ulonglong(seed))
 
     def gather(self, ptrs: Sequence[int], dst: DeviceBuffer, n_each: int) -> None:
         arr = (ctypes.c_ulonglong * len(ptrs))(*ptrs)
@@ -223,14 +228,16 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
-          seed: int = 7, log=print) -> TrainResult:
+          seed: int = 7, passes: int = 1, noise: float = 0.0, log=print) -> TrainResult:
     """Online training on trajectories of the FP32 stencil.
 
     All n_traj instances advance together through the batched stencil; after
     each 32-step advance the (state, state + 32 steps) pairs of that round are
     the training set for the next steps of Adam (minibatches of `batch`, each
-    pair used once per round).  Stops after traj_steps stencil steps or
-    max_seconds."""
+    pair used `passes` times per round, inputs perturbed by N(0, noise^2) so
+    the network learns to damp small errors of its own earlier predictions).
+    The learning rate follows a cosine from `lr` to 0 over the rounds.  Stops
+    after traj_steps stencil steps or max_seconds."""
     sur = Surrogate.get(batch)
     specs = trajectory_specs(n_traj, seed)
     st = Stencil(specs)
@@ -266,15 +273,18 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
             sur.set_scale(*scale)
             res.scale = scale
             log(f"output scale from the first round: {scale}")
-        order = rng.permutation(n_traj)
-        for j in range(0, n_traj - batch + 1, batch):
+        order = np.concatenate([rng.permutation(n_traj) for _ in range(passes)])
+        progress = r / max(1, rounds)
+        step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * progress))
+        for j in range(0, len(order) - batch + 1, batch):
             idx = order[j:j + batch]
             for q, i in enumerate(idx):
                 nb = 4 * n_each
                 sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(cur.ptr + int(i) * nb), nb, 2)
                 sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(nxt.ptr + int(i) * nb), nb, 2)
             bfk.upload(fk_host[idx])
-            step_lr = lr * (0.5 * (1 + math.cos(math.pi * min(1.0, (time.perf_counter() - t0) / max_seconds))))
+            if noise > 0.0:
+                sur.add_noise(bstate, batch * n_each, noise, seed * 1000003 + res.steps)
             loss = sur.train_step(bstate, bfk, btarget, batch, step_lr)
             res.losses.append(loss)
             res.steps += 1
@@ -316,43 +326,57 @@ class Rollout:
 
 def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarray, n_super: int,
                     check_every: int = CHECK_EVERY, bound: float = BOUND) -> Rollout:
-    """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each)
-    with the CNN; every `check_every`-th super-step is also computed by the
-    FP32 stencil from the same state, its result is kept, and if the CNN's
-    relative L2 difference on any trajectory exceeds `bound` the stencil
-    advances the rest of the rollout."""
+    """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each).
+
+    Windows of `check_every` super-steps: the CNN advances the window; its last
+    super-step is also computed by the FP32 stencil from the same state (the
+    check). If the relative L2 difference is within `bound` for every
+    trajectory, the stencil's result closes the window. Otherwise the window
+    is recomputed from its start state by the stencil, which then advances
+    the rest of the rollout."""
     B = x0.shape[0]
     nbytes = x0.nbytes
-    a, b, ref = sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes)
+    a, b, ref, ckpt = sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes)
     a.upload(x0)
     res = Rollout(state=None)
     t0 = time.perf_counter()
-    for j in range(n_super):
+    j = 0
+    while j < n_super:
+        w = min(check_every, n_super - j)
         if res.fallback_at is not None:
             st.load(sur, a, 0, B)
-            st.advance(SUPER)
+            st.advance(SUPER * w)
             st.store(sur, a, 0, B)
-            res.stencil_supersteps += 1
+            res.stencil_supersteps += w
+            j += w
             continue
-        sur.forward(a, fk, b, B)
-        res.cnn_steps += 1
-        if (j + 1) % check_every == 0:
-            st.load(sur, a, 0, B)
-            st.advance(SUPER)
-            st.store(sur, ref, 0, B)
-            res.stencil_supersteps += 1
-            p = b.download(x0.shape)
-            q = ref.download(x0.shape)
-            err = max(rel_l2(p[i], q[i]) for i in range(B))
-            res.checks.append(err)
-            a, ref = ref, a                  # keep the stencil's state
-            if err > bound:
-                res.fallback_at = j
-        else:
+        sur.lib.Sur_Copy(ctypes.c_void_p(ckpt.ptr), ctypes.c_void_p(a.ptr), ctypes.c_size_t(nbytes), 2)
+        for _ in range(w - 1):
+            sur.forward(a, fk, b, B)
             a, b = b, a
+        res.cnn_steps += w - 1
+        sur.forward(a, fk, b, B)                 # the checked super-step ...
+        res.cnn_steps += 1
+        st.load(sur, a, 0, B)                    # ... and the stencil from the same state
+        st.advance(SUPER)
+        st.store(sur, ref, 0, B)
+        res.stencil_supersteps += 1
+        p = b.download(x0.shape)
+        q = ref.download(x0.shape)
+        err = max(rel_l2(p[i], q[i]) for i in range(B))
+        res.checks.append(err)
+        if err <= bound:
+            a, ref = ref, a
+        else:                                     # roll back and redo the window with the stencil
+            res.fallback_at = j
+            st.load(sur, ckpt, 0, B)
+            st.advance(SUPER * w)
+            st.store(sur, a, 0, B)
+            res.stencil_supersteps += w
+        j += w
     res.state = a.download(x0.shape)
     res.seconds = time.perf_counter() - t0
-    for buf in (a, b, ref):
+    for buf in (a, b, ref, ckpt):
         buf.free()
     return res
 
@@ -447,13 +471,17 @@ def main(argv=None):
     ap.add_argument("--traj-steps", type=int, default=4096)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--passes", type=int, default=1)
+    ap.add_argument("--noise", type=float, default=0.0)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
     a = ap.parse_args(argv)
     if a.command == "train":
-        res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, log=lambda m: print(m, flush=True))
+        res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, passes=a.passes, noise=a.noise,
+                    log=lambda m: print(m, flush=True))
         theta = Surrogate.get(a.batch).get_params()
         meta = {"trajectories": a.trajectories, "traj_steps": a.traj_steps, "adam_steps": res.steps,
+                "passes": a.passes, "noise": a.noise, "lr": a.lr, "batch": a.batch,
                 "seconds": round(res.seconds, 1), "final_loss": float(np.mean(res.losses[-100:]))}
         save_weights(a.out, theta, res.scale, meta)
         print(json.dumps(meta), flush=True)

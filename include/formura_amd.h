@@ -49,19 +49,19 @@ float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
 
 /* Select the kernel: kind 0 = one site per thread (any stencil), kind 1 =
  * temporally blocked march with T fused steps, `rows` y-rows per lane,
- * `waves` wavefronts per tile and input planes prefetched `pf` (1 or 2)
- * planes ahead.  Returns 0, or -1 if that variant is not compiled into the
- * library. */
-int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf);
+ * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
+ * ahead, compiled for at least `occ` waves per SIMD.  Returns 0, or -1 if that
+ * variant is not compiled into the library. */
+int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
 /* x-extent of one march tile (planes per wavefront); returns 0 or -1. */
 int Formura_Set_March_Chunk(int lx);
 
 /* Current kernel selection. */
-int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *lx);
+int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *occ, int *lx);
 
-/* Writes up to max_variants (T, rows, waves, pf) quadruples of the compiled
- * march variants into out[4*i .. 4*i+3]; returns the number written. */
+/* Writes up to max_variants (T, rows, waves, pf, occ) tuples of the compiled
+ * march variants into out[5*i .. 5*i+4]; returns the number written. */
 int Formura_List_Kernels(int *out, int max_variants);
 
 /* Per-launch scalar parameters (programs compiled with named parameters);

@@ -102,12 +102,12 @@ def test_every_kernel_variant_bitwise(name, mk):
     steps = lib.layout.steps_per_forward
     st = mk(lib, 11)
     want = oracle_run(name, st, 1)
-    variants = [(KIND_DIRECT, 1, 0, 1, 1)] + [(KIND_MARCH,) + v for v in lib.list_kernels()]
-    for kind, T, by, wv, pf in variants:
+    variants = [(KIND_DIRECT, 1, 0, 1, 1, 1)] + [(KIND_MARCH,) + v for v in lib.list_kernels()]
+    for kind, T, by, wv, pf, occ in variants:
         for lx in ((64, 7) if kind == KIND_MARCH else (64,)):
-            label = f"{name} kind={kind} T={T} rows={by} waves={wv} pf={pf} lx={lx}"
+            label = f"{name} kind={kind} T={T} rows={by} waves={wv} pf={pf} occ={occ} lx={lx}"
             _say(f"[parity] {label} steps={steps}")
-            lib.set_kernel(kind, T, by, wv, pf)
+            lib.set_kernel(kind, T, by, wv, pf, occ)
             lib.set_chunk(lx)
             got = device_run(lib, st, steps)
             assert_bitwise(got, want, label)
