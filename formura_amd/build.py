@@ -8,7 +8,9 @@ reference C's statement for statement), denormals kept, packed-FP32 off.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -67,24 +69,76 @@ def generate(name: str):
     return c, h, src
 
 
-def _run(cmd: List[str]) -> None:
+def _run(cmd: List[str]) -> str:
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout[-6000:]}")
+    return r.stdout
+
+
+RESOURCE_FLAG = "-Rpass-analysis=kernel-resource-usage"
+_MARCH_NAME = re.compile(r"march3d_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)E")
+
+
+def parse_resources(text: str) -> Dict[str, Dict[str, int]]:
+    """Per-kernel VGPRs, AGPRs, ScratchSize (bytes per lane) and Occupancy
+    (waves per SIMD) from the compiler's kernel-resource-usage remarks."""
+    out: Dict[str, Dict[str, int]] = {}
+    cur = None
+    for line in text.splitlines():
+        m = re.search(r"remark: Function Name: (\S+)", line)
+        if m:
+            cur = m.group(1)
+            out[cur] = {}
+            continue
+        m = re.search(r"remark:\s+(VGPRs|AGPRs|ScratchSize|Occupancy)(?: \[[^\]]*\])?: (\d+)", line)
+        if m and cur:
+            out[cur][m.group(1)] = int(m.group(2))
+    return out
+
+
+def march_resources(res: Dict[str, Dict[str, int]]) -> Dict[tuple, Dict[str, int]]:
+    """(T, rows, waves, prefetch, occupancy) -> resources, for the march kernels."""
+    out = {}
+    for name, r in res.items():
+        m = _MARCH_NAME.search(name)
+        if m:
+            out[tuple(int(x) for x in m.groups())] = r
+    return out
 
 
 def compile_hip(src: str, out: str, extra: Optional[List[str]] = None) -> str:
     tmp = out + ".tmp"
     cmd = [HIPCC] + HIP_FLAGS + ["-I", CSRC, "-I", MPI_STUB, "-I", BUILD, "-I", INCLUDE] + (extra or []) + \
-        [src, "-o", tmp]
-    _run(cmd)
+        [RESOURCE_FLAG, src, "-o", tmp]
+    text = _run(cmd)
     os.replace(tmp, out)
+    res = march_resources(parse_resources(text))
+    with open(out[:-3] + ".resources.json", "w", encoding="utf-8") as f:
+        json.dump({",".join(map(str, k)): v for k, v in sorted(res.items())}, f, indent=0)
     return out
 
 
+def resources(name: str) -> Dict[tuple, Dict[str, int]]:
+    """Resources of the march variants compiled into lib<name>.so."""
+    with open(lib_path(name)[:-3] + ".resources.json", encoding="utf-8") as f:
+        return {tuple(int(x) for x in k.split(",")): v for k, v in json.load(f).items()}
+
+
 def build_program(name: str) -> str:
-    _, _, src = generate(name)
-    return compile_hip(src, lib_path(name))
+    """Generate and compile one program; march variants whose registers spill
+    to scratch memory are dropped and the library is compiled again without
+    them."""
+    c, _, src = generate(name)
+    out = compile_hip(src, lib_path(name))
+    spilled = [k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0]
+    if spilled:
+        from formura_amd.compiler.driver import write_hip
+        from formura_amd.compiler.emit_hip import variants_for
+        keep = [v for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision)) if tuple(v) not in spilled]
+        _, src = write_hip(c, BUILD, keep)
+        out = compile_hip(src, lib_path(name))
+    return out
 
 
 def build_copy_probe() -> str:
@@ -124,7 +178,7 @@ def build_all(names=None, jobs: int = 8) -> Dict[str, str]:
         generate(n)
     out = {}
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(compile_hip, os.path.join(BUILD, f"{n}.hip"), lib_path(n)): n for n in names}
+        futs = {ex.submit(build_program, n): n for n in names}
         for f in cf.as_completed(futs):
             out[futs[f]] = f.result()
     if "gs3d64" in names:

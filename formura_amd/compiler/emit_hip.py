@@ -21,58 +21,68 @@ from typing import Dict, List, Sequence, Tuple
 from .abi import Layout, init_body
 from .pointir import PointIR, c_expr_of, imm_literal
 
-# Register budget (32-bit registers per lane) of one march wavefront: the plane
-# windows (T levels x 3 planes), the prefetched plane and the row offsets.  A
-# wavefront at one wave per SIMD owns 256 VGPRs plus 256 AGPRs; the compiler
-# parks the overflow of the windows in AGPRs, and beyond this budget it would
-# spill to scratch memory.
-MARCH_REG_BUDGET = 340
-MAX_WAVES = 4        # <= 4 waves per workgroup keeps one wave per SIMD (512 registers)
-OCC2_REG_BUDGET = 200   # two waves per SIMD: 256 registers per wave
+# Registers (32-bit, per lane) of one march wavefront, as the compiler
+# allocates them: per row and field, the two live planes of each of the T
+# window levels, the prefetched planes and about two planes of temporaries,
+# plus a 32-bit row offset; about 20 more for the rest.  A wavefront alone on
+# its SIMD may hold 256 VGPRs plus 256 AGPRs, but every value the compiler parks
+# in an AGPR costs a move in and out, so the largest tiles stay a little above
+# 256; with two wavefronts per SIMD each holds 256 registers in all.
+# Variants that still spill to scratch memory are dropped by the build.
+AGPR_REG_BUDGET = 340    # one wave per SIMD, some window values parked in AGPRs
+VGPR_REG_BUDGET = 256    # one wave per SIMD, everything in VGPRs
+OCC2_REG_BUDGET = 240    # two waves per SIMD
+MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 
 
 def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
-    return (3 * T + pf) * by * nf * words + by
+    return by * (nf * words * (2 * T + pf + 2) + 1) + 20
+
+
+def _max_rows(T: int, nf: int, words: int, pf: int, budget: int) -> int:
+    by = 0
+    while march_regs(T, by + 1, nf, words, pf) <= budget:
+        by += 1
+    return by
 
 
 def default_march_variants(nf: int, real: str, max_t: int = None):
-    """(T, rows per wave, waves, prefetch depth) variants of the march kernel.
-
-    For each fusion depth T: a tile of up to 64 + 2T rows (64 valid rows),
-    split over the fewest waves (<= 4) whose registers fit the budget; and
-    single-wave tiles (no LDS, no barrier) with as many rows as one wave can
-    hold, prefetching one or two planes ahead."""
+    """(T, rows per wave, waves, prefetch depth, waves per SIMD) variants of
+    the march kernel, for each fusion depth T:
+      * one wave per tile (no LDS, no barrier) with as many rows as fit the
+        AGPR-assisted budget, and with as many as fit in VGPRs alone;
+      * one wave per tile at two waves per SIMD, prefetching two planes ahead;
+      * a tile of up to 64 + 2T rows split over up to 4 waves that exchange
+        their edge rows through LDS."""
     words = 2 if real == "double" else 1
     if max_t is None:
-        max_t = 4 if real == "double" else 8
+        max_t = 5 if real == "double" else 8
     out = []
     for T in range(1, max_t + 1):
-        by_max = 1
-        while march_regs(T, by_max + 1, nf, words) <= MARCH_REG_BUDGET:
-            by_max += 1
-        if by_max * MAX_WAVES < 2 * T + 2:
-            continue
-        rows = min(64 + 2 * T, by_max * MAX_WAVES)
-        W = -(-rows // by_max)
-        by = -(-rows // W)
-        if W > 1:
-            out.append((T, by, W, 1, 1))
-        by1 = min(by_max, 64 + 2 * T)
-        if by1 >= 2 * T + 2:
-            out.append((T, by1, 1, 1, 1))
-        by2 = 1
-        while march_regs(T, by2 + 1, nf, words, 2) <= MARCH_REG_BUDGET:
-            by2 += 1
-        if by2 >= 2 * T + 2:
-            out.append((T, min(by2, 64 + 2 * T), 1, 2, 1))
-        # two waves per SIMD: half the registers per wave, twice the waves
-        # to hide memory and arithmetic latency
-        by3 = 1
-        while march_regs(T, by3 + 1, nf, words, 2) <= OCC2_REG_BUDGET:
-            by3 += 1
-        if by3 >= 2 * T + 4:
-            out.append((T, by3, 1, 2, 2))
+        cand = []
+        for budget, pf, occ in ((AGPR_REG_BUDGET, 1, 1), (VGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 2, 2)):
+            by = min(_max_rows(T, nf, words, pf, budget), 64 + 2 * T)
+            if by >= 2 * T + 2:
+                cand.append((T, by, 1, pf, occ))
+        by_max = _max_rows(T, nf, words, 1, AGPR_REG_BUDGET)
+        if T >= 2 and by_max * MAX_WAVES >= 2 * T + 2:
+            rows = min(64 + 2 * T, by_max * MAX_WAVES)
+            W = -(-rows // by_max)
+            if W > 1:
+                cand.append((T, -(-rows // W), W, 1, 1))
+        for v in cand:
+            if v not in out:
+                out.append(v)
     return out
+
+
+def device_real(lay: Layout, precision: str) -> str:
+    return "float" if precision == "fp32" else lay.field_ctype[0]
+
+
+def variants_for(lay: Layout, precision: str):
+    """The march variants emit_hip compiles in by default."""
+    return default_march_variants(len(lay.field_names), device_real(lay, precision))
 
 
 def point_body(ir: PointIR, real: str) -> str:
@@ -112,7 +122,7 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     dim = lay.dim
     if dim not in (2, 3):
         raise ValueError("the HIP backend supports 2D and 3D programs")
-    real = "float" if precision == "fp32" else lay.field_ctype[0]
+    real = device_real(lay, precision)
     if precision != "fp32" and len(set(lay.field_ctype)) != 1:
         raise ValueError("all state variables must share one element type")
     r = ir.radius() + [0] * (3 - dim)
@@ -208,6 +218,12 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return ms;")
     w("}")
     w("")
+    w("int Formura_Last_Timing (float *main_ms, int *main_launches, float *total_ms)")
+    w("{")
+    w("  *main_ms = lib.last_main_ms; *main_launches = lib.last_main_launches; *total_ms = lib.last_total_ms;")
+    w("  return 0;")
+    w("}")
+    w("")
     w("int Formura_Forward (struct Formura_Navigator *navi)")
     w("{")
     w("  Formura_Upload(navi);")
@@ -227,7 +243,7 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("")
     w("int Formura_Set_March_Chunk (int lx)")
     w("{")
-    w("  if (lx < 1) return -1;")
+    w("  if (lx == 0) return -1;")
     w("  lib.lx = lx;")
     w("  return 0;")
     w("}")

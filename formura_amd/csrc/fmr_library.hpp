@@ -85,15 +85,17 @@ struct Variant {
 
 template <class P>
 using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
-                               int, Params, hipStream_t);
+                               Params, hipStream_t);
 
 template <class P, int T, int BY, int W, int PF, int OCC>
-void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int lx,
-                    int nty, int ntz, int ntx, Params prm, hipStream_t s) {
-  const int ntot = nty * ntz * ntx;
-  const int per = (ntot + N_XCD - 1) / N_XCD;
-  hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(per * N_XCD), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
-                     lx, nty, ntz, ntx, prm);
+void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
+                    int nty, int ntz, Params prm, hipStream_t s) {
+  if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
+    fprintf(stderr, "formura_amd: march kernel needs a plane (NY*NZ sites) smaller than 2 GiB\n");
+    abort();
+  }
+  hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
+                     nty, ntz, prm);
 }
 
 template <class P>
@@ -117,11 +119,12 @@ class StepLibrary {
   int nvariants = 0;
   Variant choice{KIND_DIRECT, 1, 0, 1, 1, 1};
   int lx = 64;
+  int simds = 1024;   // SIMDs of the device (CUs x 4), read at init
   double* red_part = nullptr;
   double* red_res = nullptr;
   void* staging = nullptr;
   size_t staging_bytes = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool ready = false;
 
   void init(int n0, int n1, int n2, const VariantEntry<P>* vs, int nv) {
@@ -131,6 +134,12 @@ class StepLibrary {
     variants = vs;
     nvariants = nv;
     FMR_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    {
+      int dev = 0, cus = 0;
+      FMR_CHECK(hipGetDevice(&dev));
+      FMR_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      simds = 4 * (cus > 0 ? cus : 256);
+    }
     for (int b = 0; b < 2; ++b)
       for (int f = 0; f < MAX_FIELDS; ++f) buf[b].f[f] = nullptr;
     for (int b = 0; b < 2; ++b)
@@ -141,6 +150,7 @@ class StepLibrary {
     FMR_CHECK(hipMalloc(&red_part, sizeof(double) * 3 * RED_BLOCKS));
     FMR_CHECK(hipMalloc(&red_res, sizeof(double) * 3));
     FMR_CHECK(hipEventCreate(&ev0));
+    FMR_CHECK(hipEventCreate(&evm));
     FMR_CHECK(hipEventCreate(&ev1));
     // default: the fused variant with T = 2 if compiled in, else the first one
     choice = Variant{KIND_DIRECT, 1, 0, 1, 1, 1};
@@ -206,32 +216,54 @@ class StepLibrary {
     }
   }
 
+  // workgroups of a march launch: lx > 0 asks for runs of about lx planes;
+  // lx < 0 for -lx workgroups per SIMD of the device (one run each).  Always
+  // a multiple of 8 (one share per XCD) and at most one per (tile, plane).
+  int march_groups(int nty, int ntz, int W) const {
+    const long long total = (long long)nty * ntz * n[0];
+    long long g = lx > 0 ? (total + lx - 1) / lx : (long long)(-lx) * simds / W;
+    if (g > total) g = total;
+    g = (g + N_XCD - 1) / N_XCD * N_XCD;
+    return (int)(g < N_XCD ? N_XCD : g);
+  }
+
   void launch_march(const VariantEntry<P>* e) {
     const int T = e->v.T, rows = e->v.by * e->v.waves;
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
     const int ntz = (n[2] + (WAVE - 2 * T) - 1) / (WAVE - 2 * T);
-    const int ntx = (n[0] + lx - 1) / lx;
-    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], lx, nty, ntz, ntx, prm, stream);
+    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], march_groups(nty, ntz, e->v.waves), nty, ntz, prm, stream);
     FMR_CHECK(hipGetLastError());
     cur = 1 - cur;
   }
 
-  // advance the device state by `steps` time steps (asynchronous on `stream`)
-  void advance(int steps) {
+  // the fused variant selected by `choice`, or null (direct kernel, or not compiled in)
+  const VariantEntry<P>* main_variant() const {
+    if (choice.kind == KIND_DIRECT) return nullptr;
+    return find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
+  }
+
+  // launches of the selected kernel for `steps` steps (the rest is done by advance_rest)
+  int advance_main(int steps) {
     if (choice.kind == KIND_DIRECT) {
       launch_direct(steps);
-      return;
+      return steps;
     }
-    int left = steps;
+    const VariantEntry<P>* e = main_variant();
+    if (!e) return 0;
+    int k = 0;
+    for (; (k + 1) * e->v.T <= steps; ++k) launch_march(e);
+    return k;
+  }
+
+  // `left` < T steps: the deepest variant with the same tile shape that fits,
+  // else any fused variant that fits, else the direct kernel
+  void advance_rest(int left) {
     while (left > 0) {
       const VariantEntry<P>* e = nullptr;
-      if (choice.T <= left) e = find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
-      if (!e) {   // remainder: deepest variant with the same tile shape that fits
-        for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t)
-          e = find(KIND_MARCH, t, choice.by, choice.waves, choice.pf, choice.occ);
-        for (int i = 0; i < nvariants && !e; ++i)
-          if (variants[i].v.T <= left) e = &variants[i];
-      }
+      for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t)
+        e = find(KIND_MARCH, t, choice.by, choice.waves, choice.pf, choice.occ);
+      for (int i = 0; i < nvariants && !e; ++i)
+        if (variants[i].v.T <= left) e = &variants[i];
       if (!e) {
         launch_direct(left);
         return;
@@ -241,14 +273,29 @@ class StepLibrary {
     }
   }
 
+  int steps_of_main(int launches) const {
+    const VariantEntry<P>* e = main_variant();
+    return choice.kind == KIND_DIRECT ? launches : (e ? launches * e->v.T : 0);
+  }
+
+  // advance the device state by `steps` time steps (asynchronous on `stream`)
+  void advance(int steps) { advance_rest(steps - steps_of_main(advance_main(steps))); }
+
+  // as advance, timed with events on the library stream: returns the total
+  // milliseconds; last_main_ms / last_main_launches time the selected kernel's
+  // own launches (they run first), separately from the remainder steps
+  float last_main_ms = 0.f, last_total_ms = 0.f;
+  int last_main_launches = 0;
   float timed_advance(int steps) {
     FMR_CHECK(hipEventRecord(ev0, stream));
-    advance(steps);
+    last_main_launches = advance_main(steps);
+    FMR_CHECK(hipEventRecord(evm, stream));
+    advance_rest(steps - steps_of_main(last_main_launches));
     FMR_CHECK(hipEventRecord(ev1, stream));
     FMR_CHECK(hipEventSynchronize(ev1));
-    float ms = 0.f;
-    FMR_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-    return ms;
+    FMR_CHECK(hipEventElapsedTime(&last_main_ms, ev0, evm));
+    FMR_CHECK(hipEventElapsedTime(&last_total_ms, ev0, ev1));
+    return last_total_ms;
   }
 
   void sync() { FMR_CHECK(hipStreamSynchronize(stream)); }

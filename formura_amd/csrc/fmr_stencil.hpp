@@ -60,9 +60,10 @@ __device__ __forceinline__ int wrap(int i, int n) {
 }
 
 // ---- cross-lane moves over the whole 64-lane wavefront (DPP wave_shr / wave_shl)
-// lane i receives lane i-1 (shr) or lane i+1 (shl); the lane with no source keeps 0.
-__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
-__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+// lane i receives lane i-1 (shr) or lane i+1 (shl); the lane with no source reads 0
+// (bound_ctrl set, so no zero-initialised destination register is needed).
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
 
 __device__ __forceinline__ float lane_prev(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
 __device__ __forceinline__ float lane_next(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
@@ -77,6 +78,29 @@ __device__ __forceinline__ double lane_next(double v) {
   const int lo = dpp_shl1((int)(q & 0xffffffffLL));
   const int hi = dpp_shl1((int)(q >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---- buffer loads and stores (raw buffer: uniform base in a resource, 32-bit
+// per-lane byte offset).  A resource spans one plane of a field, so the base
+// moves in scalar registers and every row keeps one 32-bit offset.
+constexpr int BUF_DWORD3 = 0x00020000;   // gfx9 raw buffer: 32-bit data format, no swizzle
+constexpr int BUF_NT = 2;                // cache policy bit: non-temporal
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, BUF_DWORD3);
+}
+__device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, BUF_NT);
+}
+__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, BUF_NT);
 }
 
 // ===========================================================================
@@ -153,12 +177,18 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // load; results are written with non-temporal stores so they do not evict
 // the ghost rows the neighbouring tiles are about to read from L2.
 //
+// Addressing: loads and stores are raw buffer operations on a resource that
+// spans one plane of a field.  The plane base lives in scalar registers and
+// moves with the march; each row keeps one 32-bit byte offset, so a row costs
+// no 64-bit address arithmetic.  A plane must be smaller than 2 GiB.
+//
 // OCC asks the compiler for at least OCC waves per SIMD (smaller register
 // budget per wave, more waves to hide latency).
 //
-// Workgroup -> tile mapping is XCD-aware: consecutive workgroups go to
-// different XCDs, so tile t is given to workgroup (t % per)*8 + t / per, which
-// keeps neighbouring tiles -- whose ghost rows overlap -- on one XCD and its L2.
+// Workgroup -> work mapping is XCD-aware: consecutive workgroups go to
+// different XCDs, so run g is given to workgroup (g % per)*8 + g / per, which
+// keeps neighbouring runs -- same tile or adjacent tiles, whose ghost rows
+// overlap -- on one XCD and its L2.
 // ===========================================================================
 template <class P, int T, int BY, int W, int PF>
 struct March3D {
@@ -167,6 +197,7 @@ struct March3D {
   static constexpr int H = T;
   static constexpr int ROWS = BY * W;
   static_assert(ROWS > 2 * T, "tile must be taller than its ghost rows");
+  static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
   using Edge = real[2][T][W][2][NF][WAVE];   // [half][level][wave][top,bottom][field][lane]
 
   real w[T][3][BY][NF];
@@ -174,6 +205,7 @@ struct March3D {
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
   int ylim;     // tile rows >= ylim lie past the grid end (periodic duplicates): never stored
+  unsigned voff[BY];   // byte offset of this lane's site of row r within a plane
 
   template <int L, int CEN, int ROW, int F>
   __device__ __forceinline__ real row_above() const {
@@ -213,12 +245,15 @@ struct March3D {
     __device__ __forceinline__ void st(real v) { out[F] = v; }
   };
 
+  // load plane x (already wrapped) into prefetch slot K
   template <int K>
-  __device__ __forceinline__ void load_plane(const Fields<real>& in, size_t xoff, const int (&yoff)[BY]) {
+  __device__ __forceinline__ void load_plane(const Fields<real>& in, int x, unsigned plane_bytes) {
 #pragma unroll
-    for (int r = 0; r < BY; ++r)
+    for (int f = 0; f < NF; ++f) {
+      const auto rs = plane_rsrc(reinterpret_cast<const char*>(in.f[f]) + (size_t)x * plane_bytes, plane_bytes);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) nxt[K][r][f] = in.f[f][xoff + yoff[r]];
+      for (int r = 0; r < BY; ++r) nxt[K][r][f] = buf_load((real*)nullptr, rs, voff[r]);
+    }
   }
 
   // rows of level L+1 computed from window L: with one wave the valid rows
@@ -228,8 +263,8 @@ struct March3D {
   static constexpr int row_hi(int L) { return W == 1 ? BY - L - 1 : BY; }
 
   template <int L, int PH, int ROW>
-  __device__ __forceinline__ void level_row(const Params& prm, int q, int y0, int zg, Fields<real>& out,
-                                            size_t qoff, const int (&yoff)[BY], bool st_ok) {
+  __device__ __forceinline__ void level_row(const Params& prm, int q, int y0, int zg, const __amdgpu_buffer_rsrc_t (&orsrc)[NF],
+                                            bool lane_ok) {
     if constexpr (ROW < row_hi(L)) {
       const int trow = wave * BY + ROW;   // row within the tile
       Acc<L, PH, ROW> a{*this, prm, q, y0 + trow, zg, {}};
@@ -238,35 +273,44 @@ struct March3D {
 #pragma unroll
         for (int f = 0; f < NF; ++f) w[L + 1][PH][ROW][f] = a.out[f];
       } else {
-        if (st_ok && trow >= T && trow < ROWS - T && trow < ylim) {
+        if (lane_ok && trow >= T && trow < ROWS - T && trow < ylim) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) __builtin_nontemporal_store(a.out[f], &out.f[f][qoff + yoff[ROW]]);
+          for (int f = 0; f < NF; ++f)
+            buf_store_nt(a.out[f], orsrc[f], voff[ROW]);
         }
       }
-      level_row<L, PH, ROW + 1>(prm, q, y0, zg, out, qoff, yoff, st_ok);
+      level_row<L, PH, ROW + 1>(prm, q, y0, zg, orsrc, lane_ok);
     }
   }
 
+  // levels L+1 .. T for one iteration.  Level T's plane q is stored when it
+  // lies in [x0, xend) and not computed otherwise (nothing else reads it).
   template <int L, int PH>
   __device__ __forceinline__ void levels(const Params& prm, int xin, int x0, int xend, int y0, int zg,
-                                         bool lane_ok, Fields<real>& out, size_t plane, const int (&yoff)[BY]) {
+                                         bool lane_ok, const Fields<real>& out, unsigned plane_bytes) {
     if constexpr (L < T) {
       const int q = xin - (L + 1);   // plane computed by level L+1
-      bool st_ok = false;
-      size_t qoff = 0;
       if constexpr (L + 1 == T) {
-        st_ok = lane_ok && q >= x0 && q < xend;
-        qoff = (size_t)(st_ok ? q : 0) * plane;
+        if (q >= x0 && q < xend) {
+          __amdgpu_buffer_rsrc_t orsrc[NF];
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
+            orsrc[f] = plane_rsrc(reinterpret_cast<char*>(out.f[f]) + (size_t)q * plane_bytes, plane_bytes);
+          level_row<L, PH, row_lo(L)>(prm, q, y0, zg, orsrc, lane_ok);
+        }
+      } else {
+        __amdgpu_buffer_rsrc_t none[NF];
+        level_row<L, PH, row_lo(L)>(prm, q, y0, zg, none, lane_ok);
+        levels<L + 1, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane_bytes);
       }
-      level_row<L, PH, row_lo(L)>(prm, q, y0, zg, out, qoff, yoff, st_ok);
-      levels<L + 1, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane, yoff);
     }
   }
 
+  // one step of the march: plane xin enters level 0, plane xl (= xin + PF,
+  // wrapped) is requested
   template <int PH>
-  __device__ __forceinline__ void iter(const Fields<real>& in, Fields<real>& out, const Params& prm, int xin,
-                                       int x0, int xend, int nx, int y0, int zg, bool lane_ok, size_t plane,
-                                       const int (&yoff)[BY]) {
+  __device__ __forceinline__ void iter(const Fields<real>& in, const Fields<real>& out, const Params& prm, int xin,
+                                       int xl, int x0, int xend, int y0, int zg, bool lane_ok, unsigned plane_bytes) {
     constexpr int CEN = (PH + 2) % 3;
 #pragma unroll
     for (int r = 0; r < BY; ++r)
@@ -287,26 +331,70 @@ struct March3D {
         }
       __syncthreads();
     }
-    load_plane<PF - 1>(in, (size_t)wrap(xin + PF, nx) * plane, yoff);
-    levels<0, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane, yoff);
+    load_plane<PF - 1>(in, xl, plane_bytes);
+    levels<0, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane_bytes);
     half ^= 1;
   }
 };
 
-template <class P, int T, int BY, int W, int PF, int OCC>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC))) march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out,
-                                                         int nx, int ny, int nz, int lx, int nty, int ntz, int ntx,
-                                                         Params prm) {
+__device__ __forceinline__ int next_plane(int x, int nx) { return x + 1 == nx ? 0 : x + 1; }
+
+// Work split: the (y, z) tiles, each with its nx planes, form one list of
+// ntile * nx (tile, plane) items, tile-major.  Workgroup g of the G launched
+// (G a multiple of 8) marches the items [g * total / G, (g + 1) * total / G):
+// a run of planes of one tile, or the end of one tile and the start of the
+// next (the pipeline restarts at a tile change).  Every workgroup gets the same
+// number of planes to +-1, so the waves of a launch finish together whatever
+// the tile count; the host picks G for a plane-run length or a number of
+// workgroups per SIMD.  A plane (ny * nz sites) must be < 2 GiB.
+template <class P, int T, int BY, int W, int PF>
+__device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fields<typename P::real>& in,
+                                          const Fields<typename P::real>& out, int nx, int ny, int nz, int ntz,
+                                          int tile, int x0, int xend, const Params& prm) {
   using M = March3D<P, T, BY, W, PF>;
+  using real = typename P::real;
   constexpr int H = T;
-  const int ntot = nty * ntz * ntx;
-  const int per = (ntot + N_XCD - 1) / N_XCD;
-  const int b = blockIdx.x;
-  const int tile = (b % N_XCD) * per + b / N_XCD;
-  if (tile >= ntot) return;   // uniform over the workgroup: no barrier is skipped by part of it
   const int tz = tile % ntz;
-  const int ty = (tile / ntz) % nty;
-  const int tx = tile / (ntz * nty);
+  const int ty = tile / ntz;
+  const int y0 = ty * (M::ROWS - 2 * H) - H;          // tile row 0
+  const int zu = tz * (WAVE - 2 * H) - H + m.lane;     // unwrapped z of this lane
+  const int zg = wrap(zu, nz);
+  const bool lane_ok = m.lane >= H && m.lane < WAVE - H && zu < nz;
+  const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
+#pragma unroll
+  for (int r = 0; r < BY; ++r)
+    m.voff[r] = (unsigned)((wrap(y0 + m.wave * BY + r, ny) * nz + zg) * sizeof(real));
+  m.ylim = ny - y0;
+  int xl = wrap(x0 - H, nx);
+  m.template load_plane<0>(in, xl, plane_bytes);
+  if constexpr (PF > 1) {
+    xl = next_plane(xl, nx);
+    m.template load_plane<PF - 1>(in, xl, plane_bytes);
+  }
+  xl = next_plane(xl, nx);   // plane requested by the first iteration: x0 - H + PF
+  const int iters = xend - x0 + 2 * T;
+  for (int s = 0; s < iters; s += 3) {
+    const int xin = x0 - H + s;
+    m.template iter<0>(in, out, prm, xin, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    xl = next_plane(xl, nx);
+    m.template iter<1>(in, out, prm, xin + 1, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    xl = next_plane(xl, nx);
+    m.template iter<2>(in, out, prm, xin + 2, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    xl = next_plane(xl, nx);
+  }
+}
+
+template <class P, int T, int BY, int W, int PF, int OCC>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
+march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int nty, int ntz,
+               Params prm) {
+  using M = March3D<P, T, BY, W, PF>;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int g = (b % N_XCD) * (G / N_XCD) + b / N_XCD;   // neighbouring runs on one XCD (shared L2)
+  const long long total = (long long)nty * ntz * nx;
+  long long a = (long long)g * total / G;
+  const long long e = (long long)(g + 1) * total / G;
   M m;
   m.wave = threadIdx.x / WAVE;
   m.lane = threadIdx.x % WAVE;
@@ -317,26 +405,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC
   } else {
     m.edge = nullptr;
   }
-  const int y0 = ty * (M::ROWS - 2 * H) - H;          // tile row 0
-  const int zu = tz * (WAVE - 2 * H) - H + m.lane;     // unwrapped z of this lane
-  const int zg = wrap(zu, nz);
-  const int x0 = tx * lx;
-  const int xend = min(x0 + lx, nx);
-  const bool lane_ok = m.lane >= H && m.lane < WAVE - H && zu < nz;
-  const size_t plane = (size_t)ny * nz;
-  int yoff[BY];   // offset of row r within a plane (a plane holds < 2^31 sites)
-#pragma unroll
-  for (int r = 0; r < BY; ++r) yoff[r] = wrap(y0 + m.wave * BY + r, ny) * nz + zg;
-  m.ylim = ny - y0;
-  m.template load_plane<0>(in, (size_t)wrap(x0 - H, nx) * plane, yoff);
-  if constexpr (PF > 1) m.template load_plane<PF - 1>(in, (size_t)wrap(x0 - H + 1, nx) * plane, yoff);
-  static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
-  const int iters = xend - x0 + 2 * T;
-  for (int s = 0; s < iters; s += 3) {
-    const int xin = x0 - H + s;
-    m.template iter<0>(in, out, prm, xin, x0, xend, nx, y0, zg, lane_ok, plane, yoff);
-    m.template iter<1>(in, out, prm, xin + 1, x0, xend, nx, y0, zg, lane_ok, plane, yoff);
-    m.template iter<2>(in, out, prm, xin + 2, x0, xend, nx, y0, zg, lane_ok, plane, yoff);
+  // a, e are uniform over the workgroup, so every wave runs the same segments
+  // and reaches the same barriers
+  while (a < e) {
+    const int tile = (int)(a / nx);
+    const int x0 = (int)(a % nx);
+    const int xend = (int)min((long long)nx, x0 + (e - a));
+    march_run<P, T, BY, W, PF>(m, in, out, nx, ny, nz, ntz, tile, x0, xend, prm);
+    a += xend - x0;
   }
 }
 

@@ -56,9 +56,10 @@ __device__ inline f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, const f32x4&
 constexpr int NL = 5;
 constexpr int CIN[NL] = {4, 48, 48, 48, 48};
 constexpr int COUT[NL] = {48, 48, 48, 48, 2};
-constexpr int CP_IN[NL] = {8, 48, 48, 48, 48};      // input channels padded to 8
+constexpr int CP_IN[NL] = {8, 48, 48, 48, 48};      // first layer: 4 features x (hi, lo)
 constexpr int CP_OUT[NL] = {48, 48, 48, 48, 8};     // output channels padded (layer 5: 8)
 constexpr int NT_OUT[NL] = {3, 3, 3, 3, 1};         // 16-wide N tiles of the forward GEMM
+static_assert(CP_IN[0] == 2 * CIN[0], "first layer: hi and lo parts of the input features");
 
 __host__ __device__ inline bf16 f2bf(float f) {   // round to nearest even
   uint32_t u;
@@ -83,7 +84,15 @@ __device__ inline float gelu_grad(float z) {
 __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ---------------------------------------------------------------------------
-// input planes -> x0 (NHWC, 8 channels, channels 4..7 zero)
+// input planes -> x0 (NHWC, 8 channels).  The four FP32 features x (u, v, F,
+// k, affinely normalised) are split into hi = bf16(x) (channels 0..3) and
+// lo = bf16(x - hi) (channels 4..7); the first layer applies the same weights
+// to both, so it sees x with about 16 significant bits.
+__device__ inline void split_bf16(float x, bf16& hi, bf16& lo) {
+  hi = f2bf(x);
+  lo = f2bf(x - bf2f(hi));   // x - hi is exact in FP32
+}
+
 __global__ void pack_input(const float* __restrict__ state, const float* __restrict__ fk, bf16* __restrict__ x0,
                            int B, int H, int W) {
   const size_t n = (size_t)B * H * W;
@@ -93,11 +102,10 @@ __global__ void pack_input(const float* __restrict__ state, const float* __restr
     const float u = state[((size_t)b * 2 + 0) * H * W + hw];
     const float v = state[((size_t)b * 2 + 1) * H * W + hw];
     bf16* o = x0 + p * 8;
-    o[0] = f2bf((u - 0.5f) * 2.0f);
-    o[1] = f2bf(v * 4.0f);
-    o[2] = f2bf((fk[2 * b] - 0.0375f) / 0.0175f);
-    o[3] = f2bf((fk[2 * b + 1] - 0.0575f) / 0.0075f);
-    o[4] = o[5] = o[6] = o[7] = 0;
+    split_bf16((u - 0.5f) * 2.0f, o[0], o[4]);
+    split_bf16(v * 4.0f, o[1], o[5]);
+    split_bf16((fk[2 * b] - 0.0375f) / 0.0175f, o[2], o[6]);
+    split_bf16((fk[2 * b + 1] - 0.0575f) / 0.0075f, o[3], o[7]);
   }
 }
 
@@ -328,17 +336,31 @@ __global__ void adam(float* __restrict__ w, const float* __restrict__ g, float* 
 
 // master [co][tap][ci] FP32 -> forward BF16 [co_pad][tap][ci_pad] and data-gradient
 // BF16 [ci_pad_n][tap][co_pad_k] (zero padded)
+// with tie != 0 (first layer, cif == 2 * ci), padded input channel c >= ci
+// takes the weight of channel c - ci (the lo parts of the split input)
 __global__ void cast_weights(const float* __restrict__ w, bf16* __restrict__ wf, bf16* __restrict__ wb, int ci,
-                             int co, int cif, int cof, int cib, int cob) {
+                             int co, int cif, int cof, int cib, int cob, int tie) {
   const int n = cof * 9 * cif;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int c = i % cif, tap = (i / cif) % 9, o = i / (cif * 9);
+    int c = i % cif;
+    const int tap = (i / cif) % 9, o = i / (cif * 9);
+    if (tie && c >= ci) c -= ci;
     wf[i] = (o < co && c < ci) ? f2bf(w[((size_t)o * 9 + tap) * ci + c]) : (bf16)0;
   }
   const int nb = cib * 9 * cob;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
     const int o = i % cob, tap = (i / cob) % 9, c = i / (cob * 9);
     wb[i] = (o < co && c < ci) ? f2bf(w[((size_t)o * 9 + tap) * ci + c]) : (bf16)0;
+  }
+}
+
+// first-layer weight gradient: g[o][tap][c] = s[o][tap][c] + s[o][tap][c + ci]
+// (s over the 2 * ci split input channels, whose two halves share weights)
+__global__ void fold_tied(const float* __restrict__ s, float* __restrict__ g, int co, int ci) {
+  const int n = co * 9 * ci;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int c = i % ci, ot = i / ci;
+    g[i] = s[(size_t)ot * 2 * ci + c] + s[(size_t)ot * 2 * ci + c + ci];
   }
 }
 
@@ -372,7 +394,7 @@ void cast_all() {
     const int cob = (l < NL - 1) ? 48 : 8;    // data-gradient K channels (layer 5: dz has 8)
     const int cib = 48;                       // data-gradient N (input channels, padded to 3 tiles)
     hipLaunchKernelGGL(cast_weights, dim3(64), dim3(256), 0, net.stream, net.w + net.off_w[l], net.wf[l], net.wb[l],
-                       CIN[l], COUT[l], CP_IN[l], NT_OUT[l] * 16, cib, cob);
+                       CIN[l], COUT[l], CP_IN[l], NT_OUT[l] * 16, cib, cob, l == 0 ? 1 : 0);
   }
   SCHECK(hipGetLastError());
 }
@@ -422,18 +444,18 @@ double backward(const float* target, int B) {
     else
       hipLaunchKernelGGL((wgrad_mfma<48, 48, 3>), dim3(wblocks), dim3(256), 0, net.stream, ain, dz, net.part, B, H,
                          W, net.RC);
-    // the first layer's partials hold columns for 8 padded input channels: sum
-    // them, then compact to the 4 real ones on the host side of the layout
+    // the first layer's partials hold columns for the 8 split input channels:
+    // sum the chunks, then fold the hi and lo halves (tied weights)
     const int ncol_pad = 9 * ci;
+    float* sum = net.part + (size_t)chunks * COUT[l] * ncol_pad;
     hipLaunchKernelGGL(sum_chunks, dim3(grid_for((size_t)COUT[l] * ncol_pad, 256)), dim3(256), 0, net.stream,
-                       net.part, net.part + (size_t)chunks * COUT[l] * ncol_pad, chunks, COUT[l] * ncol_pad);
+                       net.part, sum, chunks, COUT[l] * ncol_pad);
     if (ci == CIN[l]) {
-      SCHECK(hipMemcpyAsync(net.g + net.off_w[l], net.part + (size_t)chunks * COUT[l] * ncol_pad,
-                            sizeof(float) * COUT[l] * ncol, hipMemcpyDeviceToDevice, net.stream));
+      SCHECK(hipMemcpyAsync(net.g + net.off_w[l], sum, sizeof(float) * COUT[l] * ncol, hipMemcpyDeviceToDevice,
+                            net.stream));
     } else {
-      SCHECK(hipMemcpy2DAsync(net.g + net.off_w[l], sizeof(float) * CIN[l],
-                              net.part + (size_t)chunks * COUT[l] * ncol_pad, sizeof(float) * ci,
-                              sizeof(float) * CIN[l], (size_t)COUT[l] * 9, hipMemcpyDeviceToDevice, net.stream));
+      hipLaunchKernelGGL(fold_tied, dim3(grid_for((size_t)COUT[l] * ncol, 256)), dim3(256), 0, net.stream, sum,
+                         net.g + net.off_w[l], COUT[l], CIN[l]);
     }
     // bias gradient
     if (COUT[l] == 2)

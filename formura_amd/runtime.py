@@ -59,6 +59,8 @@ class StepLibrary:
             L.Formura_Forward_Device.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
             L.Formura_Forward_Device_Timed.argtypes = [ctypes.POINTER(self.Nav), ctypes.c_int]
             L.Formura_Forward_Device_Timed.restype = ctypes.c_float
+            L.Formura_Last_Timing.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_float)]
             L.Formura_Set_Kernel.argtypes = [ctypes.c_int] * 6
             L.Formura_Set_March_Chunk.argtypes = [ctypes.c_int]
             L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 7
@@ -108,6 +110,13 @@ class StepLibrary:
 
     def forward_device_timed(self, nsteps: int) -> float:
         return float(self.lib.Formura_Forward_Device_Timed(ctypes.byref(self.navi), int(nsteps)))
+
+    def last_timing(self):
+        """(ms of the selected kernel's launches, their count, total ms) of the
+        last forward_device_timed call."""
+        m, k, t = ctypes.c_float(), ctypes.c_int(), ctypes.c_float()
+        self.lib.Formura_Last_Timing(ctypes.byref(m), ctypes.byref(k), ctypes.byref(t))
+        return float(m.value), int(k.value), float(t.value)
 
     def set_kernel(self, kind: int, T: int = 1, rows: int = 0, waves: int = 1, pf: int = 1, occ: int = 1) -> None:
         if self.lib.Formura_Set_Kernel(kind, T, rows, waves, pf, occ) != 0:

@@ -300,6 +300,47 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     return res
 
 
+def stencil_pairs(n: int, seed: int, burn: int = 256):
+    """n held-out (state, F/k, state 32 steps later) pairs from the FP32
+    stencil, after `burn` steps from the seeded initial condition."""
+    sur = Surrogate.get()
+    specs = trajectory_specs(n, seed)
+    st = Stencil(specs)
+    x0 = initial_states(specs, seed * 1000)
+    for i in range(n):
+        st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
+    st.advance(burn)
+    buf = sur.buffer(x0.nbytes)
+    st.store(sur, buf)
+    xs = buf.download(x0.shape)
+    st.advance(SUPER)
+    st.store(sur, buf)
+    ys = buf.download(x0.shape)
+    buf.free()
+    return xs, fk_array(specs), ys
+
+
+def loss_on(theta: np.ndarray, scale, xs: np.ndarray, fk: np.ndarray, ys: np.ndarray) -> float:
+    """Mean squared error of the network with parameters theta on pairs (xs -> ys),
+    in batches of the surrogate's maximum batch (no parameter update)."""
+    sur = Surrogate.get()
+    sur.set_params(theta)
+    sur.set_scale(*scale)
+    B = sur.max_batch
+    n = xs.shape[0]
+    bs, bt, bf = sur.buffer(xs[:B].nbytes), sur.buffer(xs[:B].nbytes), sur.buffer(8 * B)
+    tot = 0.0
+    for j in range(0, n, B):
+        k = min(B, n - j)
+        bs.upload(np.ascontiguousarray(xs[j:j + k]))
+        bt.upload(np.ascontiguousarray(ys[j:j + k]))
+        bf.upload(np.ascontiguousarray(fk[j:j + k]))
+        tot += sur.gradients(bs, bf, bt, k) * k
+    for b in (bs, bt, bf):
+        b.free()
+    return tot / n
+
+
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gs2d_256.npz")
 
 

@@ -47,6 +47,11 @@ int Formura_Forward_Device(struct Formura_Navigator *navi, int nsteps);
  * with HIP events on the library's stream around the launches. */
 float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
 
+/* Breakdown of the last Formura_Forward_Device_Timed call: the selected
+ * kernel's launches run first (main_launches of them, main_ms in total), then
+ * the steps left over when nsteps is not a multiple of its fusion depth T. */
+int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
+
 /* Select the kernel: kind 0 = one site per thread (any stencil), kind 1 =
  * temporally blocked march with T fused steps, `rows` y-rows per lane,
  * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
@@ -54,7 +59,9 @@ float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
  * variant is not compiled into the library. */
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
-/* x-extent of one march tile (planes per wavefront); returns 0 or -1. */
+/* Work split of the march kernel: the (tile, plane) items are divided evenly
+ * among the launched workgroups.  lx > 0: about lx planes per workgroup;
+ * lx < 0: -lx workgroups per SIMD of the device.  Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
 
 /* Current kernel selection. */

@@ -5,7 +5,13 @@ float64 numpy, for checking the GPU kernels.
 With `quantize=True` the values are rounded to BF16 (round to nearest even)
 exactly where the GPU rounds them: the packed input planes, the weights, the
 stored pre-activations and activations, and the back-propagated
-pre-activation gradients; everything else is float64.  With `quantize=False`
+pre-activation gradients; everything else is float64.
+
+The four input features x (u, v, F, k, affinely normalised in FP32) enter the
+first layer as two BF16 parts each, hi = bf16(x) and lo = bf16(x - hi), in
+channels 0..3 and 4..7, and the first layer applies the same weights to both
+parts: W1 x is computed as W1 hi + W1 lo, so the input keeps about 16
+significant bits instead of BF16's 8 (the 32-step increment of u is ~1% of u).  With `quantize=False`
 it is the plain float64 network, whose gradients are pinned by central finite
 differences in the CPU tests.
 
@@ -48,8 +54,8 @@ def bf16(x: np.ndarray) -> np.ndarray:
 
 
 def pack_input(state: np.ndarray, fk: np.ndarray) -> np.ndarray:
-    """state [B][2][H][W], fk [B][2] -> x0 [B][H][W][4] (the GPU pack_input,
-    computed in FP32 as the GPU does, then rounded to BF16)."""
+    """state [B][2][H][W], fk [B][2] -> features [B][H][W][4], computed in
+    FP32 as the GPU's pack_input does (returned as float64)."""
     B, _, H, W = state.shape
     s = state.astype(np.float32)
     f = fk.astype(np.float32)
@@ -59,6 +65,19 @@ def pack_input(state: np.ndarray, fk: np.ndarray) -> np.ndarray:
     x[..., 2] = ((f[:, 0] - np.float32(0.0375)) / np.float32(0.0175))[:, None, None]
     x[..., 3] = ((f[:, 1] - np.float32(0.0575)) / np.float32(0.0075))[:, None, None]
     return x.astype(np.float64)
+
+
+def split_input(x: np.ndarray, quantize: bool = True) -> np.ndarray:
+    """[B][H][W][4] features -> [B][H][W][8]: hi parts then lo parts."""
+    if not quantize:
+        return np.concatenate([x, np.zeros_like(x)], axis=-1)
+    hi = bf16(x)
+    return np.concatenate([hi, bf16(x - hi)], axis=-1)
+
+
+def tie(w: np.ndarray) -> np.ndarray:
+    """First-layer weights [co][3][3][4] -> [co][3][3][8] (hi and lo parts share them)."""
+    return np.concatenate([w, w], axis=-1)
 
 
 def conv(a: np.ndarray, w: np.ndarray) -> np.ndarray:
@@ -104,11 +123,11 @@ def gelu_grad(z):
 def forward(theta, state, fk, scale, quantize=True) -> Dict:
     q = bf16 if quantize else (lambda x: x)
     layers = split(np.asarray(theta, dtype=np.float64))
-    x0 = q(pack_input(state, fk)) if quantize else pack_input(state, fk)
+    x0 = split_input(pack_input(state, fk), quantize)
     acts, zs = [x0], []
     a = x0
     for l, (w, b) in enumerate(layers):
-        z = conv(a, q(w)) + b
+        z = conv(a, q(tie(w) if l == 0 else w)) + b
         if l < len(layers) - 1:
             zs.append(q(z))
             a = q(gelu(z))
@@ -139,6 +158,8 @@ def loss_and_grad(theta, state, fk, target, scale, quantize=True):
     for l in range(len(layers) - 1, -1, -1):
         w, b = layers[l]
         gw = conv_w(acts[l], dz)
+        if l == 0:                       # tied hi / lo weights: the gradients add
+            gw = gw[..., :4] + gw[..., 4:]
         gb = dz.reshape(-1, dz.shape[-1]).sum(axis=0)
         grads.append((gw, gb))
         if l > 0:

@@ -60,3 +60,21 @@ def test_bf16_round_to_nearest_even():
     assert r[3] == -2.5
     assert r[5] == 65536.0
     assert abs(r[4] - 3.0e-30) / 3.0e-30 <= 2 ** -8
+
+
+def test_split_input_keeps_sixteen_bits():
+    """hi = bf16(x), lo = bf16(x - hi): hi + lo is x to 2^-16 (relative), both
+    parts are BF16 values, and the first layer applied to the split input with
+    tied weights is the layer applied to x itself (to the same 2^-16)."""
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1.5, 1.5, (1, 4, 5, 4)).astype(np.float32).astype(np.float64)
+    s = O.split_input(x)
+    hi, lo = s[..., :4], s[..., 4:]
+    assert np.array_equal(O.bf16(hi), hi) and np.array_equal(O.bf16(lo), lo)
+    assert np.all(np.abs(hi + lo - x) <= 2.0 ** -16 * np.abs(x))
+    assert np.max(np.abs(hi - x)) > 2.0 ** -12      # hi alone is coarser than the pair
+    w = O.bf16(rng.normal(size=(6, 3, 3, 4)))
+    full = O.conv(x, w)
+    split = O.conv(s, O.tie(w))
+    bound = 2.0 ** -16 * O.conv(np.abs(x), np.abs(w))
+    assert np.all(np.abs(split - full) <= bound + 1e-15)
