@@ -16,27 +16,28 @@ and aborts with a message on any HIP error.
 """
 from __future__ import annotations
 
+import heapq
 from typing import Dict, List, Sequence, Tuple
 
 from .abi import Layout, init_body
 from .pointir import PointIR, c_expr_of, imm_literal
+from .shiftreuse import ShiftAnalysis, depends_on, op_children
 
 # Registers (32-bit, per lane) of one march wavefront, as the compiler
 # allocates them: per row and field, the two live planes of each of the T
-# window levels, the prefetched planes and about two planes of temporaries,
-# plus a 32-bit row offset; about 20 more for the rest.  A wavefront alone on
+# window levels, the prefetched planes and about two planes of temporaries;
+# about 20 more for the rest (row offsets are scalar).  A wavefront alone on
 # its SIMD may hold 256 VGPRs plus 256 AGPRs, but every value the compiler parks
 # in an AGPR costs a move in and out, so the largest tiles stay a little above
-# 256; with two wavefronts per SIMD each holds 256 registers in all.
+# 256; within 256 registers two wavefronts share a SIMD.
 # Variants that still spill to scratch memory are dropped by the build.
 AGPR_REG_BUDGET = 340    # one wave per SIMD, some window values parked in AGPRs
-VGPR_REG_BUDGET = 256    # one wave per SIMD, everything in VGPRs
-OCC2_REG_BUDGET = 240    # two waves per SIMD
+OCC2_REG_BUDGET = 250    # two waves per SIMD (256 registers each)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 
 
 def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
-    return by * (nf * words * (2 * T + pf + 2) + 1) + 20
+    return by * nf * words * (2 * T + pf + 2) + 20
 
 
 def _max_rows(T: int, nf: int, words: int, pf: int, budget: int) -> int:
@@ -50,8 +51,9 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
     """(T, rows per wave, waves, prefetch depth, waves per SIMD) variants of
     the march kernel, for each fusion depth T:
       * one wave per tile (no LDS, no barrier) with as many rows as fit the
-        AGPR-assisted budget, and with as many as fit in VGPRs alone;
-      * one wave per tile at two waves per SIMD, prefetching two planes ahead;
+        AGPR-assisted budget, one wave per SIMD;
+      * one wave per tile at two waves per SIMD, prefetching one and two
+        planes ahead;
       * a tile of up to 64 + 2T rows split over up to 4 waves that exchange
         their edge rows through LDS."""
     words = 2 if real == "double" else 1
@@ -60,7 +62,7 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
     out = []
     for T in range(1, max_t + 1):
         cand = []
-        for budget, pf, occ in ((AGPR_REG_BUDGET, 1, 1), (VGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 2, 2)):
+        for budget, pf, occ in ((AGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 1, 2), (OCC2_REG_BUDGET, 2, 2)):
             by = min(_max_rows(T, nf, words, pf, budget), 64 + 2 * T)
             if by >= 2 * T + 2:
                 cand.append((T, by, 1, pf, occ))
@@ -85,10 +87,45 @@ def variants_for(lay: Layout, precision: str):
     return default_march_variants(len(lay.field_names), device_real(lay, precision))
 
 
+def lane_reuse(ir: PointIR, real: str):
+    """Lane-axis shift reuse: ({B: A}, op order).  B(s) = A(s - e) along the
+    contiguous axis (the march kernel's lanes); the order is the op order with
+    each such A moved before its B (a stable topological sort)."""
+    sa = ShiftAnalysis(ir)
+    lane = {b: a for b, a in sorted(sa.pairs(ir.dim - 1).items())
+            if ir.ops[b].ctype == real and not depends_on(ir, a, b)}
+    n = len(ir.ops)
+    deps = [set(op_children(op)) for op in ir.ops]
+    for b, a in lane.items():
+        deps[b].add(a)
+    users: List[List[int]] = [[] for _ in range(n)]
+    for i, ds in enumerate(deps):
+        for d in ds:
+            users[d].append(i)
+    left = [len(ds) for ds in deps]
+    ready = [i for i in range(n) if left[i] == 0]
+    heapq.heapify(ready)
+    order: List[int] = []
+    while ready:
+        i = heapq.heappop(ready)
+        order.append(i)
+        for u in users[i]:
+            left[u] -= 1
+            if left[u] == 0:
+                heapq.heappush(ready, u)
+    return lane, order
+
+
 def point_body(ir: PointIR, real: str) -> str:
+    """The per-site statements as C++ for `P::point(a)`: one `const` per op, in
+    op order except that a lane-reuse pair's A precedes its B; B is written as
+    `a.lane_prev_of(A, [&]{ return B; })` (the kernel returns the previous
+    lane's A, or evaluates B)."""
+    lane, order = lane_reuse(ir, real)
     lines: List[str] = []
     names: Dict[int, str] = {}
-    for i, op in enumerate(ir.ops):
+    for i in order:
+        op = ir.ops[i]
         v = f"a{i}"
         k = op.kind
         if k == "load":
@@ -103,7 +140,10 @@ def point_body(ir: PointIR, real: str) -> str:
         elif k == "index":
             lines.append(f"const {op.ctype} {v} = ({op.ctype})a.index({op.args[0]});")
         else:
-            lines.append(f"const {op.ctype} {v} = {c_expr_of(op, lambda j: names[j])};")
+            e = c_expr_of(op, lambda j: names[j])
+            if i in lane:
+                e = f"a.lane_prev_of({names[lane[i]]}, [&]() -> {op.ctype} {{ return {e}; }})"
+            lines.append(f"const {op.ctype} {v} = {e};")
         names[i] = v
     for f, o in enumerate(ir.outputs):
         lines.append(f"a.template st<{f}>(({real}){names[o]});")

@@ -46,7 +46,7 @@ struct March2D {
   int lane;
 
   template <int L, int PH, int V>
-  struct Acc {
+  struct Acc : EvalReuse {
     const March2D& m;
     const double* prm;
     int cx, cy;
@@ -85,7 +85,7 @@ struct March2D {
   __device__ __forceinline__ void level_point(const double* prm, int q, int ybase, real* const* out, size_t qoff,
                                               const int (&yoff)[VY], bool row_ok, int ylo, int yhi) {
     if constexpr (V < VY) {
-      Acc<L, PH, V> a{*this, prm, q, ybase + V, {}};
+      Acc<L, PH, V> a{{}, *this, prm, q, ybase + V, {}};
       P::point(a);
       if constexpr (L + 1 < T) {
 #pragma unroll
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) batched_direct_kernel(const Instance<type
   for (int i = 0; i < MAX_PARAMS; ++i) prm.v[i] = I.prm[i];
   const size_t total = (size_t)I.nx * I.ny;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    DirectAcc<P> a{in, prm, {(int)(i / I.ny), (int)(i % I.ny), 0}, {I.nx, I.ny, 1}, {}};
+    DirectAcc<P> a{{}, in, prm, {(int)(i / I.ny), (int)(i % I.ny), 0}, {I.nx, I.ny, 1}, {}};
     P::point(a);
 #pragma unroll
     for (int f = 0; f < P::NF; ++f) out.f[f][i] = a.out[f];

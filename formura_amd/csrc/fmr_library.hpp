@@ -85,17 +85,17 @@ struct Variant {
 
 template <class P>
 using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
-                               Params, hipStream_t);
+                               int, Params, hipStream_t);
 
 template <class P, int T, int BY, int W, int PF, int OCC>
 void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
-                    int nty, int ntz, Params prm, hipStream_t s) {
+                    int nty, int ntz, int chunk, Params prm, hipStream_t s) {
   if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
     fprintf(stderr, "formura_amd: march kernel needs a plane (NY*NZ sites) smaller than 2 GiB\n");
     abort();
   }
   hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
-                     nty, ntz, prm);
+                     nty, ntz, chunk, prm);
 }
 
 template <class P>
@@ -216,22 +216,32 @@ class StepLibrary {
     }
   }
 
-  // workgroups of a march launch: lx > 0 asks for runs of about lx planes;
-  // lx < 0 for -lx workgroups per SIMD of the device (one run each).  Always
-  // a multiple of 8 (one share per XCD) and at most one per (tile, plane).
-  int march_groups(int nty, int ntz, int W) const {
-    const long long total = (long long)nty * ntz * n[0];
-    long long g = lx > 0 ? (total + lx - 1) / lx : (long long)(-lx) * simds / W;
-    if (g > total) g = total;
+  // work split of a march launch (see march3d_kernel): lx > 0 cuts x into
+  // chunks of lx planes, one workgroup per (chunk, tile); lx < 0 keeps x whole
+  // and launches -lx workgroups per SIMD of the device, balanced to a plane.
+  // The workgroup count is a multiple of 8 (one share per XCD).
+  void march_split(int nty, int ntz, int W, int& groups, int& chunk) const {
+    const long long ntile = (long long)nty * ntz;
+    long long g;
+    if (lx > 0) {
+      chunk = lx < n[0] ? lx : n[0];
+      g = ntile * ((n[0] + chunk - 1) / chunk);
+    } else {
+      chunk = n[0];
+      g = (long long)(-lx) * simds / W;
+      if (g > ntile * n[0]) g = ntile * n[0];
+    }
     g = (g + N_XCD - 1) / N_XCD * N_XCD;
-    return (int)(g < N_XCD ? N_XCD : g);
+    groups = (int)(g < N_XCD ? N_XCD : g);
   }
 
   void launch_march(const VariantEntry<P>* e) {
     const int T = e->v.T, rows = e->v.by * e->v.waves;
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
     const int ntz = (n[2] + (WAVE - 2 * T) - 1) / (WAVE - 2 * T);
-    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], march_groups(nty, ntz, e->v.waves), nty, ntz, prm, stream);
+    int groups = 0, chunk = 0;
+    march_split(nty, ntz, e->v.waves, groups, chunk);
+    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], groups, nty, ntz, chunk, prm, stream);
     FMR_CHECK(hipGetLastError());
     cur = 1 - cur;
   }

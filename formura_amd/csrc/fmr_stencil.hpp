@@ -89,18 +89,19 @@ constexpr int BUF_NT = 2;                // cache policy bit: non-temporal
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, BUF_DWORD3);
 }
-__device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+// off: per-lane byte offset (VGPR); soff: uniform byte offset (SGPR)
+__device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff = 0) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
 }
-__device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+__device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff = 0) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
 }
-__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+__device__ __forceinline__ void buf_store_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff = 0) {
   using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, BUF_NT);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, soff, BUF_NT);
 }
-__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, BUF_NT);
+__device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff = 0) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, soff, BUF_NT);
 }
 
 // ===========================================================================
@@ -108,8 +109,17 @@ __device__ __forceinline__ void buf_store_nt(float v, __amdgpu_buffer_rsrc_t r, 
 // periodic index wrap.  Any stencil shape and radius; the reference kernel the
 // tiled kernels are checked against on the GPU.
 // ===========================================================================
+// Shift reuse hook called by `point` (the compiler's shift analysis finds op
+// pairs with B(s) = A(s - e) along the lane axis): a kernel that holds the
+// previous site of that axis in the neighbouring lane returns lane_prev(A);
+// the default evaluates B.
+struct EvalReuse {
+  template <class V, class Fn>
+  __device__ __forceinline__ V lane_prev_of(V, Fn f) const { return f(); }
+};
+
 template <class P>
-struct DirectAcc {
+struct DirectAcc : EvalReuse {
   using real = typename P::real;
   const Fields<real>& in;
   const Params& prm;
@@ -139,7 +149,7 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
   const size_t total = (size_t)n0 * n1 * n2;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
-    DirectAcc<P> a{in, prm, {(int)(i / ((size_t)n1 * n2)), (int)((i / n2) % n1), (int)(i % n2)}, {n0, n1, n2}, {}};
+    DirectAcc<P> a{{}, in, prm, {(int)(i / ((size_t)n1 * n2)), (int)((i / n2) % n1), (int)(i % n2)}, {n0, n1, n2}, {}};
     P::point(a);
 #pragma unroll
     for (int f = 0; f < P::NF; ++f) out.f[f][i] = a.out[f];
@@ -179,8 +189,9 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 //
 // Addressing: loads and stores are raw buffer operations on a resource that
 // spans one plane of a field.  The plane base lives in scalar registers and
-// moves with the march; each row keeps one 32-bit byte offset, so a row costs
-// no 64-bit address arithmetic.  A plane must be smaller than 2 GiB.
+// moves with the march; the row offsets (uniform) are scalar offsets and the
+// lane's z offset is one vector register, so a row costs no address
+// arithmetic and no vector register.  A plane must be smaller than 2 GiB.
 //
 // OCC asks the compiler for at least OCC waves per SIMD (smaller register
 // budget per wave, more waves to hide latency).
@@ -205,7 +216,8 @@ struct March3D {
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
   int ylim;     // tile rows >= ylim lie past the grid end (periodic duplicates): never stored
-  unsigned voff[BY];   // byte offset of this lane's site of row r within a plane
+  unsigned zoff;       // byte offset of this lane's z-column within a row
+  unsigned roff[BY];   // byte offset of row r within a plane (uniform: scalar registers)
 
   template <int L, int CEN, int ROW, int F>
   __device__ __forceinline__ real row_above() const {
@@ -243,6 +255,9 @@ struct March3D {
     __device__ __forceinline__ int index(int axis) const { return axis == 0 ? cx : (axis == 1 ? cy : cz); }
     template <int F>
     __device__ __forceinline__ void st(real v) { out[F] = v; }
+    // the lane axis is z: the previous site's value is in the previous lane
+    template <class V, class Fn>
+    __device__ __forceinline__ V lane_prev_of(V v, Fn) const { return lane_prev(v); }
   };
 
   // load plane x (already wrapped) into prefetch slot K
@@ -252,7 +267,7 @@ struct March3D {
     for (int f = 0; f < NF; ++f) {
       const auto rs = plane_rsrc(reinterpret_cast<const char*>(in.f[f]) + (size_t)x * plane_bytes, plane_bytes);
 #pragma unroll
-      for (int r = 0; r < BY; ++r) nxt[K][r][f] = buf_load((real*)nullptr, rs, voff[r]);
+      for (int r = 0; r < BY; ++r) nxt[K][r][f] = buf_load((real*)nullptr, rs, zoff, roff[r]);
     }
   }
 
@@ -276,7 +291,7 @@ struct March3D {
         if (lane_ok && trow >= T && trow < ROWS - T && trow < ylim) {
 #pragma unroll
           for (int f = 0; f < NF; ++f)
-            buf_store_nt(a.out[f], orsrc[f], voff[ROW]);
+            buf_store_nt(a.out[f], orsrc[f], zoff, roff[ROW]);
         }
       }
       level_row<L, PH, ROW + 1>(prm, q, y0, zg, orsrc, lane_ok);
@@ -339,14 +354,15 @@ struct March3D {
 
 __device__ __forceinline__ int next_plane(int x, int nx) { return x + 1 == nx ? 0 : x + 1; }
 
-// Work split: the (y, z) tiles, each with its nx planes, form one list of
-// ntile * nx (tile, plane) items, tile-major.  Workgroup g of the G launched
-// (G a multiple of 8) marches the items [g * total / G, (g + 1) * total / G):
-// a run of planes of one tile, or the end of one tile and the start of the
-// next (the pipeline restarts at a tile change).  Every workgroup gets the same
-// number of planes to +-1, so the waves of a launch finish together whatever
-// the tile count; the host picks G for a plane-run length or a number of
-// workgroups per SIMD.  A plane (ny * nz sites) must be < 2 GiB.
+// Work split: the x axis is cut into chunks of L planes (the last one may be
+// shorter) and the (chunk, tile, plane) items are listed chunk-major, so that
+// the tiles of one chunk -- whose ghost rows overlap -- are marched at the same
+// time and share their reads in L2.  Workgroup g of the G launched (G a
+// multiple of 8) marches the items [g * total / G, (g + 1) * total / G): a run
+// of planes of one (chunk, tile), or the end of one and the start of the next
+// (the pipeline restarts there).  With G = chunks x tiles every workgroup
+// marches exactly one chunk of one tile; with fewer, larger workgroup counts
+// are balanced to one plane.  A plane (ny * nz sites) must be < 2 GiB.
 template <class P, int T, int BY, int W, int PF>
 __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fields<typename P::real>& in,
                                           const Fields<typename P::real>& out, int nx, int ny, int nz, int ntz,
@@ -363,7 +379,8 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
   const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
 #pragma unroll
   for (int r = 0; r < BY; ++r)
-    m.voff[r] = (unsigned)((wrap(y0 + m.wave * BY + r, ny) * nz + zg) * sizeof(real));
+    m.roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + m.wave * BY + r, ny) * nz * (int)sizeof(real));
+  m.zoff = (unsigned)(zg * sizeof(real));
   m.ylim = ny - y0;
   int xl = wrap(x0 - H, nx);
   m.template load_plane<0>(in, xl, plane_bytes);
@@ -387,12 +404,15 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
 template <class P, int T, int BY, int W, int PF, int OCC>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int nty, int ntz,
-               Params prm) {
+               int L, Params prm) {
   using M = March3D<P, T, BY, W, PF>;
   const int G = gridDim.x;
   const int b = blockIdx.x;
   const int g = (b % N_XCD) * (G / N_XCD) + b / N_XCD;   // neighbouring runs on one XCD (shared L2)
-  const long long total = (long long)nty * ntz * nx;
+  const long long ntile = (long long)nty * ntz;
+  const long long total = ntile * nx;
+  const int nfull = nx / L;                              // chunks of exactly L planes
+  const long long full_items = ntile * nfull * L;
   long long a = (long long)g * total / G;
   const long long e = (long long)(g + 1) * total / G;
   M m;
@@ -408,9 +428,21 @@ march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx
   // a, e are uniform over the workgroup, so every wave runs the same segments
   // and reaches the same barriers
   while (a < e) {
-    const int tile = (int)(a / nx);
-    const int x0 = (int)(a % nx);
-    const int xend = (int)min((long long)nx, x0 + (e - a));
+    int c0, len;         // first plane and length of the item's chunk
+    long long r;         // item index within the chunk
+    if (a < full_items) {
+      const long long per_chunk = ntile * L;
+      c0 = (int)(a / per_chunk) * L;
+      len = L;
+      r = a % per_chunk;
+    } else {
+      c0 = nfull * L;
+      len = nx - c0;
+      r = a - full_items;
+    }
+    const int tile = (int)(r / len);
+    const int x0 = c0 + (int)(r % len);
+    const int xend = (int)min((long long)(c0 + len), x0 + (e - a));
     march_run<P, T, BY, W, PF>(m, in, out, nx, ny, nz, ntz, tile, x0, xend, prm);
     a += xend - x0;
   }

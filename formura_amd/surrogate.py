@@ -228,74 +228,88 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
-          seed: int = 7, passes: int = 1, noise: float = 0.0, log=print) -> TrainResult:
-    """Online training on trajectories of the FP32 stencil.
+          seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 96.0, log=print) -> TrainResult:
+    """Train on trajectories of the FP32 stencil: generate, then replay.
 
-    All n_traj instances advance together through the batched stencil; after
-    each 32-step advance the (state, state + 32 steps) pairs of that round are
-    the training set for the next steps of Adam (minibatches of `batch`, each
-    pair used `passes` times per round, inputs perturbed by N(0, noise^2) so
-    the network learns to damp small errors of its own earlier predictions).
-    The learning rate follows a cosine from `lr` to 0 over the rounds.  Stops
-    after traj_steps stencil steps or max_seconds."""
+    Generation: all n_traj instances advance together through the batched
+    stencil (hipGraph-captured) for traj_steps steps after a 64-step burn-in;
+    the state of every trajectory after every 32-step super-step is kept in
+    HBM (up to store_gib GiB: beyond that, only the first trajectories are
+    kept).  Training: Adam on minibatches of `batch` (state, state 32 steps
+    later) pairs drawn uniformly from all kept trajectories and times, so the
+    network sees the early, sharp states as often as the late ones; `passes`
+    x (number of pairs) / batch steps in all, the learning rate on a cosine
+    from `lr` to 0, inputs perturbed by N(0, noise^2) (targets clean) so the
+    network learns to damp small errors of its own earlier predictions.
+    Stops after max_seconds of training."""
     sur = Surrogate.get(batch)
     specs = trajectory_specs(n_traj, seed)
     st = Stencil(specs)
     n_each = 2 * H * W
+    nb = 4 * n_each                       # bytes of one state
     x0 = initial_states(specs, seed * 1000)
-    cur = sur.buffer(4 * n_each * n_traj)
-    nxt = sur.buffer(4 * n_each * n_traj)
-    fk = sur.buffer(4 * 2 * n_traj)
-    fk.upload(fk_array(specs))
     for i in range(n_traj):
         st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
-    theta = init_params(seed)
-    sur.set_params(theta)
+    rounds = (traj_steps - 64) // SUPER
+    n_keep = int(min(n_traj, store_gib * 2 ** 30 // ((rounds + 1) * nb)))
+    if n_keep < 1:
+        raise ValueError("store_gib too small for one trajectory")
     res = TrainResult()
     t0 = time.perf_counter()
-    rng = np.random.default_rng(seed)
-    # burn-in of 64 steps so the first pairs are not the raw initial condition
-    st.advance(64)
+    # generation: snaps[r] holds the n_keep kept states after super-step r
+    snaps = sur.buffer(nb * n_keep * (rounds + 1))
+    cur = sur.buffer(nb * n_traj)
+    st.advance(64)                        # burn-in: the first pairs are not the raw initial condition
     st.store(sur, cur)
-    rounds = (traj_steps - 64) // SUPER
-    bstate = sur.buffer(4 * n_each * batch)
-    btarget = sur.buffer(4 * n_each * batch)
+    sur.lib.Sur_Copy(ctypes.c_void_p(snaps.ptr), ctypes.c_void_p(cur.ptr), ctypes.c_size_t(nb * n_keep), 2)
+    for r in range(1, rounds + 1):
+        st.advance(SUPER)
+        st.store(sur, cur)
+        sur.lib.Sur_Copy(ctypes.c_void_p(snaps.ptr + r * nb * n_keep), ctypes.c_void_p(cur.ptr),
+                         ctypes.c_size_t(nb * n_keep), 2)
+    cur.free()
+    a = snaps.download((n_keep, 2, H, W))                       # round 0
+    bview = sur.buffer(nb * n_keep)
+    sur.lib.Sur_Copy(ctypes.c_void_p(bview.ptr), ctypes.c_void_p(snaps.ptr + nb * n_keep), ctypes.c_size_t(nb * n_keep), 2)
+    inc = (bview.download((n_keep, 2, H, W)) - a).astype(np.float64)
+    bview.free()
+    scale = (float(inc[:, 0].std()), float(inc[:, 1].std()))
+    sur.set_scale(*scale)
+    res.scale = scale
+    gen_s = time.perf_counter() - t0
+    log(f"generated {n_traj} trajectories x {rounds} super-steps ({n_keep} kept, "
+        f"{snaps.nbytes / 2 ** 30:.1f} GiB) in {gen_s:.1f} s; output scale {scale}")
+    # replay training
+    theta = init_params(seed)
+    sur.set_params(theta)
+    rng = np.random.default_rng(seed)
+    bstate = sur.buffer(nb * batch)
+    btarget = sur.buffer(nb * batch)
     bfk = sur.buffer(4 * 2 * batch)
     fk_host = fk_array(specs)
-    for r in range(rounds):
-        st.advance(SUPER)
-        st.store(sur, nxt)
-        if r == 0:
-            a = cur.download((n_traj, 2, H, W))
-            b = nxt.download((n_traj, 2, H, W))
-            inc = (b - a).astype(np.float64)
-            scale = (float(inc[:, 0].std()), float(inc[:, 1].std()))
-            sur.set_scale(*scale)
-            res.scale = scale
-            log(f"output scale from the first round: {scale}")
-        order = np.concatenate([rng.permutation(n_traj) for _ in range(passes)])
-        progress = r / max(1, rounds)
-        step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * progress))
-        for j in range(0, len(order) - batch + 1, batch):
-            idx = order[j:j + batch]
-            for q, i in enumerate(idx):
-                nb = 4 * n_each
-                sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(cur.ptr + int(i) * nb), nb, 2)
-                sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(nxt.ptr + int(i) * nb), nb, 2)
-            bfk.upload(fk_host[idx])
-            if noise > 0.0:
-                sur.add_noise(bstate, batch * n_each, noise, seed * 1000003 + res.steps)
-            loss = sur.train_step(bstate, bfk, btarget, batch, step_lr)
-            res.losses.append(loss)
-            res.steps += 1
-        cur, nxt = nxt, cur
-        el = time.perf_counter() - t0
-        if r % 8 == 0:
-            log(f"round {r}/{rounds}: {res.steps} Adam steps, loss {np.mean(res.losses[-50:]):.3e}, {el:.1f} s")
+    total = max(1, passes * rounds * n_keep // batch)
+    t1 = time.perf_counter()
+    for step in range(total):
+        traj = rng.integers(0, n_keep, batch)
+        rnd = rng.integers(0, rounds, batch)
+        for q in range(batch):
+            src = snaps.ptr + (int(rnd[q]) * n_keep + int(traj[q])) * nb
+            sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(src), nb, 2)
+            sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(src + n_keep * nb), nb, 2)
+        bfk.upload(fk_host[traj])
+        if noise > 0.0:
+            sur.add_noise(bstate, batch * n_each, noise, seed * 1000003 + step)
+        step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+        res.losses.append(sur.train_step(bstate, bfk, btarget, batch, step_lr))
+        res.steps += 1
+        el = time.perf_counter() - t1
+        if step % 4000 == 0:
+            log(f"step {step}/{total}: loss {np.mean(res.losses[-200:]):.3e}, lr {step_lr:.2e}, {el:.1f} s")
         if el > max_seconds:
+            log(f"stopped at step {step} of {total} (time limit)")
             break
     res.seconds = time.perf_counter() - t0
-    for buf in (cur, nxt, fk, bstate, btarget, bfk):
+    for buf in (snaps, bstate, btarget, bfk):
         buf.free()
     return res
 
@@ -455,6 +469,18 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
         ref64.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
     ref64.advance(steps, graph=False)
     truth = np.stack([np.stack(ref64.get_state(i)) for i in range(n_traj)])
+    # one super-step from the start state: the network alone vs the FP32 stencil
+    nxt_cnn = sur.buffer(x0.nbytes)
+    sur.forward(start, fk, nxt_cnn, n_traj)
+    p1 = nxt_cnn.download(x0.shape)
+    st.load(sur, start)
+    st.advance(SUPER)
+    st.store(sur, nxt_cnn)
+    q1 = nxt_cnn.download(x0.shape)
+    nxt_cnn.free()
+    errs = [rel_l2(p1[i], q1[i]) for i in range(n_traj)]
+    out["one_step_rel_l2"] = {"max": max(errs), "mean": float(np.mean(errs))}
+    log(f"one super-step, network vs stencil: {out['one_step_rel_l2']}")
     # checked surrogate (warm-up first: one forward and one stencil super-step)
     st.load(sur, start)
     st.advance(SUPER)

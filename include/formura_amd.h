@@ -59,9 +59,10 @@ int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
  * variant is not compiled into the library. */
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
-/* Work split of the march kernel: the (tile, plane) items are divided evenly
- * among the launched workgroups.  lx > 0: about lx planes per workgroup;
- * lx < 0: -lx workgroups per SIMD of the device.  Returns 0, or -1 for 0. */
+/* Work split of the march kernel.  lx > 0: x is cut into chunks of lx
+ * planes, one workgroup per (chunk, tile); lx < 0: x is kept whole and -lx
+ * workgroups per SIMD of the device share the (tile, plane) items evenly.
+ * Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
 
 /* Current kernel selection. */
