@@ -6,12 +6,14 @@ u_t = D lap(u), 7-point stencil, 512^3 periodic grid, FP64, D=1, dx=1, dt=0.1,
 initial u i.i.d. N(0,1) -- compiled from examples/heat-3d-512.fmr into a HIP
 step library.  A "step" is one time step over the whole grid; `value` is
 lattice-site updates per second (GLUP/s) with the state resident in HBM.
-The kernel variant (fusion depth T, tile shape, march chunk) is chosen by a
-short timed search before the timed region; the best rate for each fusion depth
-T in {1, 2, 4, 8} is reported as well.
+The kernel variant (fusion depth T, tile shape, work split) is chosen by a
+short timed search before the timed region; the best rate for each compiled
+fusion depth T is reported as well (t_sweep_glups).
 
 Also reported (same line, key "extra"): the FP32 bulk heat rate, 3D Gray-Scott
-512^3 in FP64 and FP32, and 3D Gray-Scott 64^3 FP64 (1000 steps).
+512^3 in FP64 and FP32, 3D Gray-Scott 64^3 FP64 (1000 steps), the batched 2D
+Gray-Scott sweep (256 uneven instances x 5000 steps, FP32 vs FP64 vs the plain
+method) and the checked surrogate on held-out 256^2 trajectories.
 
 cpu_baseline: the reference-faithful C/OpenMP oracle library of the same
 program, built with gcc -O3 -fopenmp -ffp-contract=off, on all host cores, on
@@ -168,13 +170,17 @@ def main():
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     s0, _ = lib.reduce(0)
 
+    # HBM bytes per launch of this kernel variant, from rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs scaled by the copy-kernel calibration (profiles/); null
+    # for a variant that was not profiled
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_heat512.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
-        if t.get("variant") == list(var) + [lx]:
-            traffic = t.get("hbm_bytes_per_launch")
+        rec = t.get("variants", {}).get(",".join(str(x) for x in list(var) + [lx]))
+        if rec:
+            traffic = rec["hbm_bytes_per_launch"]
 
     out = {
         "metric": "GLUP/s & %HBM roofline; batched/surrogate speedup at <=1e-3 rel L2 vs FP64",
@@ -219,6 +225,17 @@ def main():
                             "best_per_T": {T: round(x[0], 1) for T, x in sorted(best.items())}}
             log(f"{label}: {extra[label]}")
         out["extra"] = extra
+        # batched Gray-Scott sweep: 256 uneven 2D instances x 5000 steps, FP32
+        # bulk (hipGraph, autotuned plan) vs its FP64 twin and the plain method
+        from formura_amd.batched import sweep_benchmark
+        log("batched sweep ...")
+        extra["batched_sweep"] = sweep_benchmark(log=log)
+        # checked surrogate on held-out 256^2 trajectories (committed weights)
+        from formura_amd import surrogate as S
+        if os.path.exists(S.WEIGHTS):
+            log("checked surrogate ...")
+            theta, scale, _ = S.load_weights()
+            extra["surrogate"] = S.evaluate(theta, scale, log=log)
 
     if not a.no_cpu_baseline:
         log("cpu baseline ...")

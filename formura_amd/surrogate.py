@@ -377,10 +377,11 @@ class Rollout:
     checks: List[float] = field(default_factory=list)
     fallback_at: Optional[int] = None
     seconds: float = 0.0
+    windows: List[np.ndarray] = field(default_factory=list)   # state after each window (keep_windows)
 
 
 def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarray, n_super: int,
-                    check_every: int = CHECK_EVERY, bound: float = BOUND) -> Rollout:
+                    check_every: int = CHECK_EVERY, bound: float = BOUND, keep_windows: bool = False) -> Rollout:
     """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each).
 
     Windows of `check_every` super-steps: the CNN advances the window; its last
@@ -404,6 +405,8 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
             st.store(sur, a, 0, B)
             res.stencil_supersteps += w
             j += w
+            if keep_windows:
+                res.windows.append(a.download(x0.shape))
             continue
         sur.lib.Sur_Copy(ctypes.c_void_p(ckpt.ptr), ctypes.c_void_p(a.ptr), ctypes.c_size_t(nbytes), 2)
         for _ in range(w - 1):
@@ -429,6 +432,8 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
             st.store(sur, a, 0, B)
             res.stencil_supersteps += w
         j += w
+        if keep_windows:
+            res.windows.append(a.download(x0.shape))
     res.state = a.download(x0.shape)
     res.seconds = time.perf_counter() - t0
     for buf in (a, b, ref, ckpt):
@@ -467,8 +472,11 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     ref64.plan([0] * n_traj, [(2, 2, 32)])
     for i in range(n_traj):
         ref64.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
-    ref64.advance(steps, graph=False)
-    truth = np.stack([np.stack(ref64.get_state(i)) for i in range(n_traj)])
+    truths = []
+    for j in range(0, n_super, CHECK_EVERY):
+        ref64.advance(SUPER * min(CHECK_EVERY, n_super - j), graph=False)
+        truths.append(np.stack([np.stack(ref64.get_state(i)) for i in range(n_traj)]))
+    truth = truths[-1]
     # one super-step from the start state: the network alone vs the FP32 stencil
     nxt_cnn = sur.buffer(x0.nbytes)
     sur.forward(start, fk, nxt_cnn, n_traj)
@@ -489,6 +497,13 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
                         "stencil_supersteps": ro.stencil_supersteps, "fallback_at": ro.fallback_at,
                         "check_errors": [float(f"{e:.3e}") for e in ro.checks],
                         "rel_l2_vs_fp64": rel_l2(ro.state.astype(np.float64), truth)}
+    # the same rollout again (untimed), keeping the state after each window,
+    # for the error against the FP64 run as the rollout goes on
+    st.load(sur, start)
+    ro2 = checked_rollout(sur, st, fk, xs, n_super, bound=bound, keep_windows=True)
+    out["surrogate"]["rel_l2_vs_fp64_by_window"] = [
+        float(f"{rel_l2(w.astype(np.float64), t):.3e}") for w, t in zip(ro2.windows, truths)]
+    out["surrogate"]["steps_by_window"] = [SUPER * min(n_super, j + CHECK_EVERY) for j in range(0, n_super, CHECK_EVERY)]
     log(f"checked surrogate: {out['surrogate']}")
     # plain stencils and the batched stencil from the same state
     for name, key in (("gs2d_batch_f32", "plain_fp32"), ("gs2d_batch_f64", "plain_fp64")):

@@ -90,3 +90,37 @@ def test_training_reduces_loss():
     after = S.loss_on(trained, res.scale, xs, fk, ys)
     _say(f"[surrogate] held-out loss {before:.3e} -> {after:.3e} after {res.steps} Adam steps")
     assert after < 0.5 * before
+
+
+def test_trained_network_passes_the_checks_the_untrained_one_fails():
+    """What the trained network gains over the method without it: with the
+    committed weights (formura_amd/data/surrogate_gs2d_256.npz) every check of
+    a checked rollout on held-out trajectories passes, so the network advances
+    all 16 super-steps (512 stencil steps) and the stencil runs only the 2
+    checked super-steps; an untrained network fails the first check and the
+    rollout falls back to the stencil for everything."""
+    from formura_amd import surrogate as S
+    theta, scale, _ = S.load_weights()
+    sur = S.Surrogate.get()
+    specs = S.trajectory_specs(8, seed=99)
+    x0 = S.initial_states(specs, 99000)
+    st = S.Stencil(specs)
+    for i in range(len(specs)):
+        st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
+    st.advance(256)
+    buf = sur.buffer(x0.nbytes)
+    st.store(sur, buf)
+    xs = buf.download(x0.shape)
+    buf.free()
+    fk = sur.buffer(8 * len(specs))
+    fk.upload(S.fk_array(specs))
+    sur.set_params(theta)
+    sur.set_scale(*scale)
+    ro = S.checked_rollout(sur, st, fk, xs, 16)
+    _say(f"[surrogate] trained: checks {ro.checks}, fallback {ro.fallback_at}, network super-steps {ro.cnn_steps}")
+    assert ro.fallback_at is None and ro.cnn_steps == 16 and max(ro.checks) <= S.BOUND
+    sur.set_params(S.init_params(7))
+    ro0 = S.checked_rollout(sur, st, fk, xs, 16)
+    _say(f"[surrogate] untrained: checks {ro0.checks}, fallback {ro0.fallback_at}")
+    assert ro0.fallback_at == 0 and ro0.checks[0] > S.BOUND
+    fk.free()
