@@ -26,18 +26,18 @@ from .shiftreuse import ShiftAnalysis, depends_on, op_children
 # Registers (32-bit, per lane) of one march wavefront, as the compiler
 # allocates them: per row and field, the two live planes of each of the T
 # window levels, the prefetched planes and about two planes of temporaries;
-# about 20 more for the rest (row offsets are scalar).  A wavefront alone on
+# about 24 more for the rest (row offsets are scalar).  A wavefront alone on
 # its SIMD may hold 256 VGPRs plus 256 AGPRs, but every value the compiler parks
 # in an AGPR costs a move in and out, so the largest tiles stay a little above
 # 256; within 256 registers two wavefronts share a SIMD.
 # Variants that still spill to scratch memory are dropped by the build.
 AGPR_REG_BUDGET = 340    # one wave per SIMD, some window values parked in AGPRs
-OCC2_REG_BUDGET = 250    # two waves per SIMD (256 registers each)
+OCC2_REG_BUDGET = 236    # two waves per SIMD (256 registers each, some slack for the scheduler)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 
 
 def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
-    return by * nf * words * (2 * T + pf + 2) + 20
+    return by * nf * words * (2 * T + pf + 2) + 24
 
 
 def _max_rows(T: int, nf: int, words: int, pf: int, budget: int) -> int:

@@ -10,9 +10,7 @@
 // Tile = one 64-lane wavefront covering 64*VY consecutive y-points (VY per
 // lane) and marching along x over LX rows, with T time levels pipelined in a
 // 3-row register window per level, as the 3D march kernel does one dimension
-// up.  Rows are read and written with raw buffer operations on a resource
-// spanning one row (uniform base, per-lane 32-bit offsets); the last level is
-// computed only for the rows it stores.  Neighbours: x +-1 from the window, y +-1 from the lane's own registers
+// up.  Neighbours: x +-1 from the window, y +-1 from the lane's own registers
 // or, across lanes, by DPP wave shifts.  Ghost zones of T points at both y
 // ends and T rows before and after the chunk are recomputed redundantly; reads
 // wrap periodically.  The per-site arithmetic is the program's `point`, so an
@@ -75,22 +73,17 @@ struct March2D {
     __device__ __forceinline__ void st(real v) { out[F] = v; }
   };
 
-  // row xrow (wrapped) of every field into the prefetch registers: raw buffer
-  // loads on a resource spanning the row (uniform base), per-lane byte offsets
-  __device__ __forceinline__ void load_row(real* const* in, int xrow, int ny, const unsigned (&yoff)[VY]) {
-    const unsigned row_bytes = (unsigned)ny * sizeof(real);
+  __device__ __forceinline__ void load_row(real* const* in, int xrow, int ny, const int (&yoff)[VY]) {
+    const size_t base = (size_t)xrow * ny;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      const auto rs = plane_rsrc(in[f] + (size_t)xrow * ny, row_bytes);
+    for (int v = 0; v < VY; ++v)
 #pragma unroll
-      for (int v = 0; v < VY; ++v) nxt[v][f] = buf_load((real*)nullptr, rs, yoff[v]);
-    }
+      for (int f = 0; f < NF; ++f) nxt[v][f] = in[f][base + yoff[v]];
   }
 
   template <int L, int PH, int V>
-  __device__ __forceinline__ void level_point(const double* prm, int q, int ybase,
-                                              const __amdgpu_buffer_rsrc_t (&orow)[NF], const unsigned (&yoff)[VY],
-                                              bool row_ok, int ylo, int yhi) {
+  __device__ __forceinline__ void level_point(const double* prm, int q, int ybase, real* const* out, size_t qoff,
+                                              const int (&yoff)[VY], bool row_ok, int ylo, int yhi) {
     if constexpr (V < VY) {
       Acc<L, PH, V> a{{}, *this, prm, q, ybase + V, {}};
       P::point(a);
@@ -101,36 +94,32 @@ struct March2D {
         const int t = lane * VY + V;   // position within the tile
         if (row_ok && t >= ylo && t < yhi) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) buf_store_nt(a.out[f], orow[f], yoff[V]);
+          for (int f = 0; f < NF; ++f) __builtin_nontemporal_store(a.out[f], &out[f][qoff + yoff[V]]);
         }
       }
-      level_point<L, PH, V + 1>(prm, q, ybase, orow, yoff, row_ok, ylo, yhi);
+      level_point<L, PH, V + 1>(prm, q, ybase, out, qoff, yoff, row_ok, ylo, yhi);
     }
   }
 
   template <int L, int PH>
   __device__ __forceinline__ void levels(const double* prm, int xin, int x0, int xend, int ybase, real* const* out,
-                                         int ny, const unsigned (&yoff)[VY], int ylo, int yhi) {
+                                         int ny, const int (&yoff)[VY], int ylo, int yhi) {
     if constexpr (L < T) {
       const int q = xin - (L + 1);
-      __amdgpu_buffer_rsrc_t orow[NF];
+      bool row_ok = false;
+      size_t qoff = 0;
       if constexpr (L + 1 == T) {
-        // the last level is computed only for the rows it stores
-        if (q >= x0 && q < xend) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) orow[f] = plane_rsrc(out[f] + (size_t)q * ny, (unsigned)ny * sizeof(real));
-          level_point<L, PH, 0>(prm, q, ybase, orow, yoff, true, ylo, yhi);
-        }
-      } else {
-        level_point<L, PH, 0>(prm, q, ybase, orow, yoff, false, ylo, yhi);
-        levels<L + 1, PH>(prm, xin, x0, xend, ybase, out, ny, yoff, ylo, yhi);
+        row_ok = q >= x0 && q < xend;
+        qoff = (size_t)(row_ok ? q : 0) * ny;
       }
+      level_point<L, PH, 0>(prm, q, ybase, out, qoff, yoff, row_ok, ylo, yhi);
+      levels<L + 1, PH>(prm, xin, x0, xend, ybase, out, ny, yoff, ylo, yhi);
     }
   }
 
   template <int PH>
   __device__ __forceinline__ void iter(real* const* in, real* const* out, const double* prm, int xin, int x0,
-                                       int xend, int nx, int ny, int ybase, const unsigned (&yoff)[VY], int ylo,
+                                       int xend, int nx, int ny, int ybase, const int (&yoff)[VY], int ylo,
                                        int yhi) {
 #pragma unroll
     for (int v = 0; v < VY; ++v)
@@ -160,19 +149,14 @@ __global__ void __launch_bounds__(64) batched_march2d_kernel(const Instance<type
     const int xend = min(x0 + lx, nx);
     const int ystart = tl.y0 - T;                 // tile position 0
     const int ybase = ystart + m.lane * VY;       // unwrapped y of this lane's first point
-    unsigned yoff[VY];   // byte offsets of this lane's points within a row
+    int yoff[VY];
 #pragma unroll
-    for (int v = 0; v < VY; ++v) yoff[v] = (unsigned)(wrap(ybase + v, ny) * sizeof(real));
+    for (int v = 0; v < VY; ++v) yoff[v] = wrap(ybase + v, ny);
     // positions [T, WY - T) are valid; positions past the grid end are
     // periodic duplicates of the first tile and are not stored
     const int ylo = T;
     const int yhi = min(M::WY - T, ny - ystart);
-    // parameters in registers for the whole tile (the buffer stores could
-    // alias the descriptor table, so reads through I.prm would be repeated)
-    double prm_l[MAX_PARAMS];
-#pragma unroll
-    for (int i = 0; i < MAX_PARAMS; ++i) prm_l[i] = I.prm[i];
-    const double* prm = prm_l;
+    const double* prm = I.prm;
     m.load_row(in, wrap(x0 - T, nx), ny, yoff);
     const int iters = xend - x0 + 2 * T;
     for (int s = 0; s < iters; s += 3) {

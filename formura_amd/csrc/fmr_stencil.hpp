@@ -85,6 +85,10 @@ __device__ __forceinline__ double lane_next(double v) {
 // moves in scalar registers and every row keeps one 32-bit offset.
 constexpr int BUF_DWORD3 = 0x00020000;   // gfx9 raw buffer: 32-bit data format, no swizzle
 constexpr int BUF_NT = 2;                // cache policy bit: non-temporal
+// A vector offset >= the resource's record count makes a raw buffer access
+// out of range: a load returns 0, a store is dropped.  Resources here span at
+// most 2 GiB, so this offset masks a lane without a branch.
+constexpr unsigned BUF_OOB = 0x80000000u;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, BUF_DWORD3);
@@ -215,8 +219,8 @@ struct March3D {
   real nxt[PF][BY][NF];   // planes prefetched PF iterations ahead
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
-  int ylim;     // tile rows >= ylim lie past the grid end (periodic duplicates): never stored
   unsigned zoff;       // byte offset of this lane's z-column within a row
+  unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane (the store is dropped)
   unsigned roff[BY];   // byte offset of row r within a plane (uniform: scalar registers)
 
   template <int L, int CEN, int ROW, int F>
@@ -271,30 +275,41 @@ struct March3D {
     }
   }
 
-  // rows of level L+1 computed from window L: with one wave the valid rows
-  // [L+1, BY-L-1) are known at compile time; with several, every row is
-  // computed and the stores filter.
+  // rows of level L+1 computed from window L: the tile's rows [L+1,
+  // ROWS-L-1).  With one wave the range is known at compile time; with
+  // several, the first and last wave skip their share by a uniform test on
+  // the rows near their edge (the middle rows need none).
   static constexpr int row_lo(int L) { return W == 1 ? L + 1 : 0; }
   static constexpr int row_hi(int L) { return W == 1 ? BY - L - 1 : BY; }
 
+  template <int L, int ROW>
+  __device__ __forceinline__ bool row_needed() const {
+    if constexpr (W == 1 || (ROW >= L + 1 && ROW < BY - L - 1)) return true;
+    else return (wave > 0 || ROW >= L + 1) && (wave < W - 1 || ROW < BY - L - 1);
+  }
+
+  // Stores of level T: every valid row (rows [T, ROWS-T)) is written; ghost
+  // lanes and lanes past the grid end are masked by an out-of-range offset.
+  // Rows past the grid end (last y-tile) are periodic duplicates of rows
+  // another tile owns -- the same operations on the same values, so the same
+  // bits -- and are written too (identical values, in any order).
   template <int L, int PH, int ROW>
   __device__ __forceinline__ void level_row(const Params& prm, int q, int y0, int zg, const __amdgpu_buffer_rsrc_t (&orsrc)[NF],
-                                            bool lane_ok) {
+                                            bool st_ok) {
     if constexpr (ROW < row_hi(L)) {
-      const int trow = wave * BY + ROW;   // row within the tile
-      Acc<L, PH, ROW> a{*this, prm, q, y0 + trow, zg, {}};
-      P::point(a);
-      if constexpr (L + 1 < T) {
+      if (row_needed<L, ROW>()) {
+        const int trow = wave * BY + ROW;   // row within the tile
+        Acc<L, PH, ROW> a{*this, prm, q, y0 + trow, zg, {}};
+        P::point(a);
+        if constexpr (L + 1 < T) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) w[L + 1][PH][ROW][f] = a.out[f];
-      } else {
-        if (lane_ok && trow >= T && trow < ROWS - T && trow < ylim) {
+          for (int f = 0; f < NF; ++f) w[L + 1][PH][ROW][f] = a.out[f];
+        } else if (st_ok && (W == 1 || (trow >= T && trow < ROWS - T))) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f)
-            buf_store_nt(a.out[f], orsrc[f], zoff, roff[ROW]);
+          for (int f = 0; f < NF; ++f) buf_store_nt(a.out[f], orsrc[f], zst, roff[ROW]);
         }
       }
-      level_row<L, PH, ROW + 1>(prm, q, y0, zg, orsrc, lane_ok);
+      level_row<L, PH, ROW + 1>(prm, q, y0, zg, orsrc, st_ok);
     }
   }
 
@@ -302,7 +317,7 @@ struct March3D {
   // lies in [x0, xend) and not computed otherwise (nothing else reads it).
   template <int L, int PH>
   __device__ __forceinline__ void levels(const Params& prm, int xin, int x0, int xend, int y0, int zg,
-                                         bool lane_ok, const Fields<real>& out, unsigned plane_bytes) {
+                                         const Fields<real>& out, unsigned plane_bytes) {
     if constexpr (L < T) {
       const int q = xin - (L + 1);   // plane computed by level L+1
       if constexpr (L + 1 == T) {
@@ -311,12 +326,12 @@ struct March3D {
 #pragma unroll
           for (int f = 0; f < NF; ++f)
             orsrc[f] = plane_rsrc(reinterpret_cast<char*>(out.f[f]) + (size_t)q * plane_bytes, plane_bytes);
-          level_row<L, PH, row_lo(L)>(prm, q, y0, zg, orsrc, lane_ok);
+          level_row<L, PH, row_lo(L)>(prm, q, y0, zg, orsrc, true);
         }
       } else {
         __amdgpu_buffer_rsrc_t none[NF];
-        level_row<L, PH, row_lo(L)>(prm, q, y0, zg, none, lane_ok);
-        levels<L + 1, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane_bytes);
+        level_row<L, PH, row_lo(L)>(prm, q, y0, zg, none, false);
+        levels<L + 1, PH>(prm, xin, x0, xend, y0, zg, out, plane_bytes);
       }
     }
   }
@@ -325,7 +340,7 @@ struct March3D {
   // wrapped) is requested
   template <int PH>
   __device__ __forceinline__ void iter(const Fields<real>& in, const Fields<real>& out, const Params& prm, int xin,
-                                       int xl, int x0, int xend, int y0, int zg, bool lane_ok, unsigned plane_bytes) {
+                                       int xl, int x0, int xend, int y0, int zg, unsigned plane_bytes) {
     constexpr int CEN = (PH + 2) % 3;
 #pragma unroll
     for (int r = 0; r < BY; ++r)
@@ -347,7 +362,7 @@ struct March3D {
       __syncthreads();
     }
     load_plane<PF - 1>(in, xl, plane_bytes);
-    levels<0, PH>(prm, xin, x0, xend, y0, zg, lane_ok, out, plane_bytes);
+    levels<0, PH>(prm, xin, x0, xend, y0, zg, out, plane_bytes);
     half ^= 1;
   }
 };
@@ -375,13 +390,13 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
   const int y0 = ty * (M::ROWS - 2 * H) - H;          // tile row 0
   const int zu = tz * (WAVE - 2 * H) - H + m.lane;     // unwrapped z of this lane
   const int zg = wrap(zu, nz);
-  const bool lane_ok = m.lane >= H && m.lane < WAVE - H && zu < nz;
+
   const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
 #pragma unroll
   for (int r = 0; r < BY; ++r)
     m.roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + m.wave * BY + r, ny) * nz * (int)sizeof(real));
   m.zoff = (unsigned)(zg * sizeof(real));
-  m.ylim = ny - y0;
+  m.zst = (m.lane >= H && m.lane < WAVE - H && zu < nz) ? m.zoff : BUF_OOB;   // lanes past the grid end: duplicates, not written
   int xl = wrap(x0 - H, nx);
   m.template load_plane<0>(in, xl, plane_bytes);
   if constexpr (PF > 1) {
@@ -392,11 +407,11 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
   const int iters = xend - x0 + 2 * T;
   for (int s = 0; s < iters; s += 3) {
     const int xin = x0 - H + s;
-    m.template iter<0>(in, out, prm, xin, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    m.template iter<0>(in, out, prm, xin, xl, x0, xend, y0, zg, plane_bytes);
     xl = next_plane(xl, nx);
-    m.template iter<1>(in, out, prm, xin + 1, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    m.template iter<1>(in, out, prm, xin + 1, xl, x0, xend, y0, zg, plane_bytes);
     xl = next_plane(xl, nx);
-    m.template iter<2>(in, out, prm, xin + 2, xl, x0, xend, y0, zg, lane_ok, plane_bytes);
+    m.template iter<2>(in, out, prm, xin + 2, xl, x0, xend, y0, zg, plane_bytes);
     xl = next_plane(xl, nx);
   }
 }

@@ -228,7 +228,8 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
-          seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 96.0, log=print) -> TrainResult:
+          seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 96.0, unroll: int = 0,
+          log=print) -> TrainResult:
     """Train on trajectories of the FP32 stencil: generate, then replay.
 
     Generation: all n_traj instances advance together through the batched
@@ -241,6 +242,10 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     x (number of pairs) / batch steps in all, the learning rate on a cosine
     from `lr` to 0, inputs perturbed by N(0, noise^2) (targets clean) so the
     network learns to damp small errors of its own earlier predictions.
+    With unroll > 0 (pushforward training), each minibatch first goes
+    through j = 0..unroll passes of the network itself (no gradient), and the
+    target is the true state j + 1 super-steps after the start: the network
+    learns to steer its own drifted states back to the trajectory.
     Stops after max_seconds of training."""
     sur = Surrogate.get(batch)
     specs = trajectory_specs(n_traj, seed)
@@ -284,19 +289,26 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     sur.set_params(theta)
     rng = np.random.default_rng(seed)
     bstate = sur.buffer(nb * batch)
+    bspare = sur.buffer(nb * batch) if unroll > 0 else None
     btarget = sur.buffer(nb * batch)
     bfk = sur.buffer(4 * 2 * batch)
     fk_host = fk_array(specs)
     total = max(1, passes * rounds * n_keep // batch)
     t1 = time.perf_counter()
     for step in range(total):
+        # pushforward from the second quarter on (the untrained network's passes are noise)
+        j = int(rng.integers(0, unroll + 1)) if unroll > 0 and step >= total // 4 else 0
         traj = rng.integers(0, n_keep, batch)
-        rnd = rng.integers(0, rounds, batch)
+        rnd = rng.integers(0, rounds - j, batch)
         for q in range(batch):
             src = snaps.ptr + (int(rnd[q]) * n_keep + int(traj[q])) * nb
             sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(src), nb, 2)
-            sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(src + n_keep * nb), nb, 2)
+            sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(src + (j + 1) * n_keep * nb),
+                             nb, 2)
         bfk.upload(fk_host[traj])
+        for _ in range(j):                 # the network's own passes, no gradient
+            sur.forward(bstate, bfk, bspare, batch)
+            bstate, bspare = bspare, bstate
         if noise > 0.0:
             sur.add_noise(bstate, batch * n_each, noise, seed * 1000003 + step)
         step_lr = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
@@ -309,8 +321,9 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
             log(f"stopped at step {step} of {total} (time limit)")
             break
     res.seconds = time.perf_counter() - t0
-    for buf in (snaps, bstate, btarget, bfk):
-        buf.free()
+    for buf in (snaps, bstate, bspare, btarget, bfk):
+        if buf is not None:
+            buf.free()
     return res
 
 
@@ -555,15 +568,16 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--passes", type=int, default=1)
     ap.add_argument("--noise", type=float, default=0.0)
+    ap.add_argument("--unroll", type=int, default=0, help="pushforward: up to this many network passes before the target")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
     a = ap.parse_args(argv)
     if a.command == "train":
         res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, passes=a.passes, noise=a.noise,
-                    log=lambda m: print(m, flush=True))
+                    unroll=a.unroll, log=lambda m: print(m, flush=True))
         theta = Surrogate.get(a.batch).get_params()
         meta = {"trajectories": a.trajectories, "traj_steps": a.traj_steps, "adam_steps": res.steps,
-                "passes": a.passes, "noise": a.noise, "lr": a.lr, "batch": a.batch,
+                "passes": a.passes, "noise": a.noise, "unroll": a.unroll, "lr": a.lr, "batch": a.batch,
                 "seconds": round(res.seconds, 1), "final_loss": float(np.mean(res.losses[-100:]))}
         save_weights(a.out, theta, res.scale, meta)
         print(json.dumps(meta), flush=True)
