@@ -49,8 +49,10 @@ def seeded_state(lib, seed):
 
 
 def candidates(lib):
-    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH
+    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED
     out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)]
+    if lib.num_stages() > 0:
+        out.append(((KIND_STAGED, 1, 0, 1, 1, 1), 64))
     for v in lib.list_kernels():
         for lx in (32, 64, 128, -1, -2):   # run length, or -workgroups per SIMD
             out.append(((KIND_MARCH,) + tuple(v), lx))
@@ -211,18 +213,25 @@ def main():
     if not a.no_extra:
         extra = {}
         for name, label in (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp64"),
-                            ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64")):
+                            ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64"),
+                            ("heat_rk2", "heat256_midpoint_fp64"), ("gs3d_rk4", "grayscott64_rk4_fp64")):
             lg = load_program(name)
             lg.init()
             lg.set_state(seeded_state(lg, 1))
             lg.upload()
-            best = autotune(lg, probe_steps=24 if "512" in name else 200, final_steps=96 if "512" in name else 1000)
+            big = "512" in name or "rk" in name
+            best = autotune(lg, probe_steps=24 if big else 200, final_steps=96 if big else 1000)
             r, v, l = max(best.values())
-            steps = 100 if "512" in name else 1000
+            steps = 100 if "512" in name or "rk" in name else 1000
             w, _, _, _ = timed_run(lg, v, l, steps, 10)
             n = int(np.prod(lg.layout.intra))
             extra[label] = {"glups": round(steps * n / w / 1e9, 2), "kernel": list(v) + [l],
                             "best_per_T": {T: round(x[0], 1) for T, x in sorted(best.items())}}
+            if lg.num_stages() > 0:   # multi-stage program: stage kernels vs the inlined point function
+                from formura_amd.runtime import KIND_DIRECT, KIND_STAGED
+                for kind, key in ((KIND_STAGED, "staged_glups"), (KIND_DIRECT, "inlined_direct_glups")):
+                    w2, _, _, _ = timed_run(lg, (kind, 1, 0, 1, 1, 1), 64, 20, 2)
+                    extra[label][key] = round(20 * n / w2 / 1e9, 2)
             log(f"{label}: {extra[label]}")
         out["extra"] = extra
         # batched Gray-Scott sweep: 256 uneven 2D instances x 5000 steps, FP32

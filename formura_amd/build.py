@@ -53,6 +53,13 @@ PROGRAMS: Dict[str, tuple] = {
     "gs3d_t": (_ex("gray-scott-3d-64.fmr"), _ex("gray-scott-3d-64.yaml"), "faithful", (),
                {"intra_node_shape": [36, 70, 130], "monitor_interval": 20}, None),
     "gs2d": (_ex("gray-scott-2d.fmr"), _ex("gray-scott-2d.yaml"), "faithful", (), None, None),
+    # multi-stage programs (manifest nodes): explicit midpoint heat, RK4 Gray-Scott
+    "heat_rk2": (_ex("heat-3d-rk2.fmr"), _ex("heat-3d-rk2.yaml"), "faithful", (), None, None),
+    "heat_rk2_t": (_ex("heat-3d-rk2.fmr"), _ex("heat-3d-rk2.yaml"), "faithful", (),
+                   {"intra_node_shape": [24, 40, 36], "monitor_interval": 4}, None),
+    "gs3d_rk4": (_ex("gray-scott-3d-rk4.fmr"), _ex("gray-scott-3d-rk4.yaml"), "faithful", (), None, None),
+    "gs3d_rk4_t": (_ex("gray-scott-3d-rk4.fmr"), _ex("gray-scott-3d-rk4.yaml"), "faithful", (),
+                   {"intra_node_shape": [20, 34, 40], "monitor_interval": 4}, None),
 }
 
 

@@ -15,7 +15,7 @@ from .manifest import MMProgram, gen_mm_program
 from .numerical_config import NumericalConfig
 from .omgraph import OMProgram, gen_om_program
 from .parser import parse_program
-from .pointir import PointIR, lower_step
+from .pointir import lower_stages, PointIR, lower_step
 
 
 @dataclass
@@ -53,7 +53,8 @@ def write_hip(c: Compiled, outdir: str, march_variants=None) -> Tuple[str, str]:
     h = os.path.join(outdir, f"{c.name}.h")
     src = os.path.join(outdir, f"{c.name}.hip")
     _write_if_changed(h, header_text(c.layout))
-    _write_if_changed(src, emit_hip(c.layout, c.ir, c.precision, march_variants))
+    stages = lower_stages(c.mm, c.precision, c.ir.params)
+    _write_if_changed(src, emit_hip(c.layout, c.ir, c.precision, march_variants, stages))
     return h, src
 
 

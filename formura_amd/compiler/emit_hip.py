@@ -20,7 +20,7 @@ import heapq
 from typing import Dict, List, Sequence, Tuple
 
 from .abi import Layout, init_body
-from .pointir import PointIR, c_expr_of, imm_literal
+from .pointir import PointIR, Stage, c_expr_of, imm_literal
 from .shiftreuse import ShiftAnalysis, depends_on, op_children
 
 # Registers (32-bit, per lane) of one march wavefront, as the compiler
@@ -158,7 +158,10 @@ def is_star(ir: PointIR) -> bool:
 
 
 def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
-             march_variants: Sequence[Tuple[int, int]] = None) -> str:
+             march_variants: Sequence[Tuple[int, int]] = None, stages: Sequence[Stage] = ()) -> str:
+    """HIP library source of one program: the inlined point function (direct
+    and march kernels) and, for a multi-stage program, one stage kernel per
+    manifest node (`stages`, from lower_stages)."""
     dim = lay.dim
     if dim not in (2, 3):
         raise ValueError("the HIP backend supports 2D and 3D programs")
@@ -197,6 +200,28 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  }")
     w("};")
     w("")
+    stages = [st for st in stages] if stages and any(st.target == "temp" for st in stages) else []
+    if stages and any(t != real for t in stages[0].ir.field_ctypes):
+        stages = []       # stage arrays of another type than the state: inlined kernels only
+    ntemps = sum(1 for st in stages if st.target == "temp")
+    for k, st in enumerate(stages):
+        w(f"// manifest node {st.om_id} -> {'state field ' + lay.field_names[st.index] if st.target == 'state' else 'stage array ' + str(st.index)}")
+        w(f"struct Stage{k} {{")
+        w(f"  using real = {real};")
+        w(f"  static constexpr int DIM = {dim};")
+        w("  static constexpr int NF = 1;")
+        w("  template <class A>")
+        w("  __device__ __forceinline__ static void point(A& a) {")
+        w("    " + point_body(st.ir, real))
+        w("  }")
+        w("};")
+        w("")
+    if stages:
+        w("const fmr::StageEntry<Prog> kStages[] = {")
+        for k, st in enumerate(stages):
+            w(f"  {{&fmr::launch_stage<Prog, Stage{k}>, {1 if st.target == 'state' else 0}, {st.index}}},")
+        w("};")
+        w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
     for T, by, wv, pf, occ in variants:
         w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}, {occ}}}, "
@@ -221,6 +246,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("{")
     w("  " + init_body(lay))
     w(f"  lib.init({n[0]}, {n[1]}, {n[2]}, kVariants, kNumVariants);")
+    if stages:
+        w(f"  lib.init_stages(kStages, {len(stages)}, {ntemps});")
     w("  return 0;")
     w("}")
     w("")
@@ -276,9 +303,19 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("int Formura_Set_Kernel (int kind, int T, int rows, int waves, int pf, int occ)")
     w("{")
     w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}; return 0; }")
+    w("  if (kind == fmr::KIND_STAGED) {")
+    w("    if (lib.nstages == 0) return -1;")
+    w("    lib.choice = fmr::Variant{fmr::KIND_STAGED, 1, 0, 1, 1, 1};")
+    w("    return 0;")
+    w("  }")
     w("  if (!lib.find(kind, T, rows, waves, pf, occ)) return -1;")
     w("  lib.choice = fmr::Variant{kind, T, rows, waves, pf, occ};")
     w("  return 0;")
+    w("}")
+    w("")
+    w("int Formura_Num_Stages (void)")
+    w("{")
+    w("  return lib.nstages;")
     w("}")
     w("")
     w("int Formura_Set_March_Chunk (int lx)")

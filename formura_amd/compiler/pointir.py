@@ -160,6 +160,94 @@ def lower_step(om: OMProgram, precision: str = "faithful",
     return PointIR(dim, list(om.axes), extents, fields, field_ctypes, ops, outputs, param_list)
 
 
+@dataclass
+class Stage:
+    """One manifest node of the step, evaluated over the whole grid by one
+    kernel: `ir` has one output; its loads index the arrays [state fields...,
+    stage arrays...]; the result goes to stage array `index` (target
+    'temp') or to the new value of state field `index` (target 'state')."""
+    ir: PointIR
+    target: str
+    index: int
+    om_id: int
+
+
+def stage_order(mm) -> Tuple[List[int], List[int]]:
+    """Manifest nodes in evaluation order -- the non-Store ones in MM order,
+    then the Stores (the order the CPU oracle uses)."""
+    temps, stores = [], []
+    for k, n in mm.step.items():
+        (stores if n.micro[n.root()].inst[0] == "Store" else temps).append(k)
+    return temps, stores
+
+
+def lower_stages(mm, precision: str = "faithful", params: Sequence[str] = ()) -> List[Stage]:
+    """The step as a sequence of manifest-node kernels, as the reference
+    evaluates it (one loop nest per manifest node, MPICxx/Translate.hs:671-737):
+    each node's micro DAG (Manifestation.hs:93-158) becomes a point function
+    whose `LoadCursorStatic` reads a state field and whose `LoadCursor` reads
+    the array of an earlier manifest node.  Same operations as the inlined
+    step, in stages: storing a value of type T and reading it back changes no
+    bit, so the results are the same."""
+    ctypes = C_TYPES if precision == "faithful" else C_TYPES_FP32
+    fields = list(mm.state_order)
+    nf = len(fields)
+    temps, stores = stage_order(mm)
+    temp_index = {k: nf + i for i, k in enumerate(temps)}
+    g = mm.om.step_graph
+    param_list = list(params)
+    param_nodes = {nid: param_list.index(n.name) for nid, n in g.items()
+                   if n.name in param_list and n.typ[0] == "elem"}
+    field_ctypes = [ctypes[mm.state_signature[f][2]] for f in fields]
+    for k in temps:
+        t = mm.step[k].typ
+        field_ctypes.append(ctypes.get(t[2] if t[0] == "grid" else t[1], "double"))
+    names = fields + [f"Om_{k}" for k in temps]
+    nc = mm.numerical_config
+    extents = nc.global_extents() if nc is not None else []
+    out: List[Stage] = []
+    for k in temps + stores:
+        n = mm.step[k]
+        ops: List[Op] = []
+        ref: Dict[int, int] = {}
+        for j in sorted(n.micro):
+            mn = n.micro[j]
+            inst = mn.inst
+            tag = inst[0]
+            ct = ctypes.get(mn.elem, "double")
+            if mn.om_id in param_nodes and tag not in ("LoadCursorStatic", "LoadCursor", "Store"):
+                op = Op("param", (param_nodes[mn.om_id],), ct)
+            elif tag == "LoadCursorStatic":
+                op = Op("load", (fields.index(inst[2]), tuple(inst[1])), ct)
+            elif tag == "LoadCursor":
+                op = Op("load", (temp_index[inst[2]], tuple(inst[1])), ct)
+            elif tag == "Imm":
+                op = Op("imm", (inst[1],), ct)
+            elif tag == "LoadIndex":
+                op = Op("index", (inst[1],), ct)
+            elif tag == "Store":
+                ref[j] = ref[inst[2]]
+                continue
+            elif tag == "Uniop":
+                op = Op("uniop", (inst[1], ref[inst[2]]), ct)
+            elif tag == "Binop":
+                op = Op("binop", (inst[1], ref[inst[2]], ref[inst[3]]), ct)
+            elif tag == "Triop":
+                op = Op("triop", (inst[1], ref[inst[2]], ref[inst[3]], ref[inst[4]]), ct)
+            elif tag == "Naryop":
+                op = Op("naryop", (inst[1], [ref[x] for x in inst[2]]), ct)
+            else:
+                raise FormuraCompileError(f"cannot lower micro node {tag}")
+            ref[j] = len(ops)
+            ops.append(op)
+        ir = PointIR(mm.dimension, list(mm.axes), extents, names, field_ctypes, ops, [ref[n.root()]], param_list)
+        if k in temp_index:
+            out.append(Stage(ir, "temp", temp_index[k], k))
+        else:
+            out.append(Stage(ir, "state", fields.index(n.micro[n.root()].inst[1]), k))
+    return out
+
+
 def imm_literal(r: Fraction, ctype: str) -> str:
     """C literal of a rational: the nearest double (MPICxx/Translate.hs:585,
     `realToFrac r :: Double` is correctly rounded), printed shortest

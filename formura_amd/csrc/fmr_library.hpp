@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(64) reduce_pass2(const double* part, double* r
 }
 
 // ---- kernel variants --------------------------------------------------------
-enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1 };
+enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2 };
 
 struct Variant {
   int kind;
@@ -96,6 +96,23 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
   }
   hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
                      nty, ntz, chunk, prm);
+}
+
+// one manifest node of a multi-stage step (stage_kernel<S>)
+template <class P>
+struct StageEntry {
+  void (*fn)(const Arrays<typename P::real>&, typename P::real*, int, int, int, const Params&, hipStream_t);
+  int to_state;   // 1: writes the new value of state field `index`; 0: stage array `index`
+  int index;      // in the array list: state fields first, then stage arrays
+};
+
+template <class P, class S>
+void launch_stage(const Arrays<typename P::real>& in, typename P::real* out, int n0, int n1, int n2,
+                  const Params& prm, hipStream_t s) {
+  const size_t count = (size_t)n0 * n1 * n2;
+  const size_t need = (count + 255) / 256;
+  const int blocks = (int)(need < 65536 ? need : 65536);
+  hipLaunchKernelGGL((stage_kernel<S>), dim3(blocks), dim3(256), 0, s, in, out, n0, n1, n2, prm);
 }
 
 template <class P>
@@ -125,6 +142,11 @@ class StepLibrary {
   void* staging = nullptr;
   size_t staging_bytes = 0;
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
+  // multi-stage programs: the manifest-node kernels and their arrays
+  const StageEntry<P>* stages = nullptr;
+  int nstages = 0;
+  int ntemps = 0;
+  real* temps[MAX_ARRAYS] = {};
   bool ready = false;
 
   void init(int n0, int n1, int n2, const VariantEntry<P>* vs, int nv) {
@@ -158,6 +180,39 @@ class StepLibrary {
       if (choice.kind == KIND_DIRECT || (variants[i].v.T == 2 && choice.T != 2)) choice = variants[i].v;
     FMR_CHECK(hipStreamSynchronize(stream));
     ready = true;
+  }
+
+  // the manifest-node kernels of a multi-stage step, with ntemp stage arrays
+  void init_stages(const StageEntry<P>* st, int ns, int nt) {
+    if (P::NF + nt > MAX_ARRAYS) {
+      fprintf(stderr, "formura_amd: %d state fields + %d stage arrays exceed %d\n", P::NF, nt, MAX_ARRAYS);
+      abort();
+    }
+    stages = st;
+    nstages = ns;
+    ntemps = nt;
+    for (int t = 0; t < nt; ++t) {
+      FMR_CHECK(hipMalloc(&temps[t], count * sizeof(real)));
+      FMR_CHECK(hipMemsetAsync(temps[t], 0, count * sizeof(real), stream));
+    }
+    FMR_CHECK(hipStreamSynchronize(stream));
+  }
+
+  // one step as the sequence of manifest-node kernels
+  void launch_staged(int steps) {
+    for (int s = 0; s < steps; ++s) {
+      Arrays<real> a;
+      for (int i = 0; i < MAX_ARRAYS; ++i) a.a[i] = nullptr;
+      for (int f = 0; f < P::NF; ++f) a.a[f] = buf[cur].f[f];
+      for (int t = 0; t < ntemps; ++t) a.a[P::NF + t] = temps[t];
+      for (int k = 0; k < nstages; ++k) {
+        const StageEntry<P>& e = stages[k];
+        real* out = e.to_state ? buf[1 - cur].f[e.index] : temps[e.index - P::NF];
+        e.fn(a, out, n[0], n[1], n[2], prm, stream);
+        FMR_CHECK(hipGetLastError());
+      }
+      cur = 1 - cur;
+    }
   }
 
   void* host_staging(size_t bytes) {
@@ -248,7 +303,7 @@ class StepLibrary {
 
   // the fused variant selected by `choice`, or null (direct kernel, or not compiled in)
   const VariantEntry<P>* main_variant() const {
-    if (choice.kind == KIND_DIRECT) return nullptr;
+    if (choice.kind != KIND_MARCH) return nullptr;
     return find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
   }
 
@@ -256,6 +311,10 @@ class StepLibrary {
   int advance_main(int steps) {
     if (choice.kind == KIND_DIRECT) {
       launch_direct(steps);
+      return steps;
+    }
+    if (choice.kind == KIND_STAGED) {
+      launch_staged(steps);
       return steps;
     }
     const VariantEntry<P>* e = main_variant();
@@ -285,7 +344,8 @@ class StepLibrary {
 
   int steps_of_main(int launches) const {
     const VariantEntry<P>* e = main_variant();
-    return choice.kind == KIND_DIRECT ? launches : (e ? launches * e->v.T : 0);
+    if (choice.kind == KIND_DIRECT || choice.kind == KIND_STAGED) return launches;
+    return e ? launches * e->v.T : 0;
   }
 
   // advance the device state by `steps` time steps (asynchronous on `stream`)

@@ -161,6 +161,57 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 }
 
 // ===========================================================================
+// Kernel 1b: one manifest node of a multi-stage step (a program with
+// `manifest ::` declarations, e.g. Runge-Kutta stages), one site per thread.
+// `S::point` reads the state fields and the arrays of earlier manifest nodes
+// through `a.template ld<ARRAY, D0, D1, D2>()` (ARRAY indexes in.a: state
+// fields first, then the stage arrays) and writes one value with
+// `a.template st<0>(v)`.  Each node is evaluated once per site and stored, as
+// the reference's loop nest per manifest node does, instead of being
+// recomputed inside its users.
+// ===========================================================================
+constexpr int MAX_ARRAYS = 64;
+
+template <class real>
+struct Arrays {
+  real* a[MAX_ARRAYS];
+};
+
+template <class S>
+struct StageAcc : EvalReuse {
+  using real = typename S::real;
+  const Arrays<real>& in;
+  const Params& prm;
+  int c[3];
+  int n[3];
+  real out[1];
+
+  template <int F, int D0, int D1, int D2>
+  __device__ __forceinline__ real ld() const {
+    const int x = (D0 == 0) ? c[0] : wrap(c[0] + D0, n[0]);
+    const int y = (D1 == 0) ? c[1] : wrap(c[1] + D1, n[1]);
+    const int z = (D2 == 0) ? c[2] : wrap(c[2] + D2, n[2]);
+    return in.a[F][((size_t)x * n[1] + y) * n[2] + z];
+  }
+  __device__ __forceinline__ double param(int i) const { return prm.v[i]; }
+  __device__ __forceinline__ int index(int axis) const { return c[axis]; }
+  template <int F>
+  __device__ __forceinline__ void st(real v) { out[F] = v; }
+};
+
+template <class S>
+__global__ void __launch_bounds__(256) stage_kernel(Arrays<typename S::real> in, typename S::real* out, int n0,
+                                                    int n1, int n2, Params prm) {
+  const size_t total = (size_t)n0 * n1 * n2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    StageAcc<S> a{{}, in, prm, {(int)(i / ((size_t)n1 * n2)), (int)((i / n2) % n1), (int)(i % n2)}, {n0, n1, n2}, {}};
+    S::point(a);
+    out[i] = a.out[0];
+  }
+}
+
+// ===========================================================================
 // Kernel 2: 3D temporally blocked march (star stencils of radius 1).
 //
 // A workgroup of W wavefronts owns a tile of 64 z-columns (one per lane; z is

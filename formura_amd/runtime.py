@@ -65,6 +65,7 @@ class StepLibrary:
             L.Formura_Set_March_Chunk.argtypes = [ctypes.c_int]
             L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 7
             L.Formura_List_Kernels.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+            L.Formura_Num_Stages.restype = ctypes.c_int
             L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.Formura_Device_Field.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.restype = ctypes.c_void_p
@@ -110,6 +111,10 @@ class StepLibrary:
 
     def forward_device_timed(self, nsteps: int) -> float:
         return float(self.lib.Formura_Forward_Device_Timed(ctypes.byref(self.navi), int(nsteps)))
+
+    def num_stages(self) -> int:
+        """Manifest-node kernels of a multi-stage program (0: single stage)."""
+        return int(self.lib.Formura_Num_Stages())
 
     def last_timing(self):
         """(ms of the selected kernel's launches, their count, total ms) of the
@@ -167,6 +172,7 @@ class StepLibrary:
 
 KIND_DIRECT = 0
 KIND_MARCH = 1
+KIND_STAGED = 2      # multi-stage programs: one kernel per manifest node
 
 
 def load_program(name: str) -> StepLibrary:

@@ -52,8 +52,10 @@ float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
  * the steps left over when nsteps is not a multiple of its fusion depth T. */
 int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
 
-/* Select the kernel: kind 0 = one site per thread (any stencil), kind 1 =
- * temporally blocked march with T fused steps, `rows` y-rows per lane,
+/* Select the kernel: kind 0 = one site per thread (any stencil), kind 2 =
+ * one kernel per manifest node of a multi-stage program (the reference's loop
+ * nest per manifest node, MPICxx/Translate.hs:671-737; other arguments
+ * ignored), kind 1 = temporally blocked march with T fused steps, `rows` y-rows per lane,
  * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
  * ahead, compiled for at least `occ` waves per SIMD.  Returns 0, or -1 if that
  * variant is not compiled into the library. */
@@ -64,6 +66,10 @@ int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
  * workgroups per SIMD of the device share the (tile, plane) items evenly.
  * Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
+
+/* Number of manifest-node kernels (kind 2) compiled into the library; 0 for a
+ * program whose only manifest nodes are the state updates. */
+int Formura_Num_Stages(void);
 
 /* Current kernel selection. */
 int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *occ, int *lx);

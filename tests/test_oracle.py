@@ -215,3 +215,51 @@ def test_parameterised_oracle_equals_baked_constants():
     assert baked.time_step == param.time_step == 160
     a, b = param.get_state(), baked.get_state()
     assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["V"], b["V"])
+
+
+def test_heat_rk2_oracle_fourier_mode(tmp_path):
+    """Two-stage program (the half step is a manifest node): a Fourier mode is
+    scaled per step by the explicit midpoint method's factor 1 + z + z^2/2,
+    z = c * sum_a (2 cos(2 pi k_a / n_a) - 2), c = dt D / dx^2 = 0.1."""
+    n = (16, 8, 12)
+    steps = 6
+    lib, c = build_oracle_variant(tmp_path, "heatrk2", "heat-3d-rk2.fmr", n, steps)
+    assert sum(1 for k in c.mm.step.values() if k.micro[k.root()].inst[0] != "Store") == 1
+    k = (1, 2, 3)
+    grids = np.meshgrid(*[np.arange(s) for s in n], indexing="ij")
+    u0 = np.cos(sum(2 * np.pi * kk * g / s for kk, g, s in zip(k, grids, n)))
+    lib.set_state({"u": u0})
+    lib.forward()
+    z = 0.1 * sum(2.0 * np.cos(2 * np.pi * kk / s) - 2.0 for kk, s in zip(k, n))
+    want = u0 * (1.0 + z + 0.5 * z * z) ** steps
+    assert np.abs(lib.get_state()["u"] - want).max() < 1e-13
+
+
+def test_gs_rk4_oracle_equals_numpy_restatement(tmp_path):
+    """Classical RK4 with six manifest stage arrays, restated in numpy in the
+    program's association order: bit-equal."""
+    n = (10, 12, 8)
+    lib, c = build_oracle_variant(tmp_path, "gsrk4", "gray-scott-3d-rk4.fmr", n, 4)
+    u, v = gray_scott_ic(n, 5)
+    lib.set_state({"U": u, "V": v})
+    lib.forward()
+    got = lib.get_state()
+    cu = 2e-5 / ((1.0 / 64.0) * (1.0 / 64.0))
+    cv = 1e-5 / ((1.0 / 64.0) * (1.0 / 64.0))
+    dt = 2.0
+
+    def ddt(a, b):
+        return (((-((a * b) * b)) + (0.04 * (1.0 - a))) + (cu * lap3(a)),
+                (((a * b) * b) - ((0.04 + 0.06) * b)) + (cv * lap3(b)))
+
+    for _ in range(4):
+        k1 = ddt(u, v)
+        u2, v2 = u + (dt / 2.0) * k1[0], v + (dt / 2.0) * k1[1]
+        k2 = ddt(u2, v2)
+        u3, v3 = u + (dt / 2.0) * k2[0], v + (dt / 2.0) * k2[1]
+        k3 = ddt(u3, v3)
+        u4, v4 = u + dt * k3[0], v + dt * k3[1]
+        k4 = ddt(u4, v4)
+        u = u + (dt / 6.0) * (((k1[0] + 2.0 * k2[0]) + 2.0 * k3[0]) + k4[0])
+        v = v + (dt / 6.0) * (((k1[1] + 2.0 * k2[1]) + 2.0 * k3[1]) + k4[1])
+    assert np.array_equal(got["U"], u) and np.array_equal(got["V"], v)
