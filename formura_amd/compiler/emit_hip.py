@@ -32,6 +32,9 @@ from .shiftreuse import ShiftAnalysis, depends_on, op_children
 # 256; within 256 registers two wavefronts share a SIMD.
 # Variants that still spill to scratch memory are dropped by the build.
 AGPR_REG_BUDGET = 340    # one wave per SIMD, some window values parked in AGPRs
+# A multi-stage step whose inlined point function would exceed this many
+# operations per site gets stage kernels only (3D MHD inlined: ~550 000).
+INLINE_LIMIT = 6000
 OCC2_REG_BUDGET = 236    # two waves per SIMD (256 registers each, some slack for the scheduler)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 
@@ -170,7 +173,12 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
         raise ValueError("all state variables must share one element type")
     r = ir.radius() + [0] * (3 - dim)
     star = is_star(ir)
-    use_march = dim == 3 and star and max(r) <= 1
+    stages = [st for st in stages] if stages and any(st.target == "temp" for st in stages) else []
+    if stages and any(t != real for t in stages[0].ir.field_ctypes):
+        stages = []       # stage arrays of another type than the state: inlined kernels only
+    has_stages = bool(stages)
+    inlined = len(ir.ops) <= INLINE_LIMIT or not has_stages
+    use_march = dim == 3 and star and max(r) <= 1 and inlined
     if march_variants is None:
         march_variants = default_march_variants(len(lay.field_names), real)
     variants = list(march_variants) if use_march else []
@@ -194,15 +202,17 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w(f"  static constexpr int NF = {len(lay.field_names)};")
     w(f"  static constexpr int RX = {r[0]}, RY = {r[1]}, RZ = {r[2]};")
     w(f"  static constexpr bool STAR = {'true' if star else 'false'};")
-    w("  template <class A>")
-    w("  __device__ __forceinline__ static void point(A& a) {")
-    w("    " + point_body(ir, real))
-    w("  }")
+    w(f"  static constexpr bool INLINED = {'true' if inlined else 'false'};")
+    if inlined:
+        w("  template <class A>")
+        w("  __device__ __forceinline__ static void point(A& a) {")
+        w("    " + point_body(ir, real))
+        w("  }")
+    else:
+        w(f"  // the step inlined into one point function would take {len(ir.ops)} operations per")
+        w("  // site (every manifest node recomputed inside its users): stage kernels only")
     w("};")
     w("")
-    stages = [st for st in stages] if stages and any(st.target == "temp" for st in stages) else []
-    if stages and any(t != real for t in stages[0].ir.field_ctypes):
-        stages = []       # stage arrays of another type than the state: inlined kernels only
     ntemps = sum(1 for st in stages if st.target == "temp")
     for k, st in enumerate(stages):
         w(f"// manifest node {st.om_id} -> {'state field ' + lay.field_names[st.index] if st.target == 'state' else 'stage array ' + str(st.index)}")
@@ -302,7 +312,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("")
     w("int Formura_Set_Kernel (int kind, int T, int rows, int waves, int pf, int occ)")
     w("{")
-    w("  if (kind == fmr::KIND_DIRECT) { lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}; return 0; }")
+    w("  if (kind == fmr::KIND_DIRECT) {")
+    w("    if (!Prog::INLINED) return -1;")
+    w("    lib.choice = fmr::Variant{fmr::KIND_DIRECT, 1, 0, 1, 1, 1};")
+    w("    return 0;")
+    w("  }")
     w("  if (kind == fmr::KIND_STAGED) {")
     w("    if (lib.nstages == 0) return -1;")
     w("    lib.choice = fmr::Variant{fmr::KIND_STAGED, 1, 0, 1, 1, 1};")

@@ -404,8 +404,9 @@ class _Parser:
             k = self.const_rational_expr()
             if s == "-":
                 k = -k
-        elif not name:
+        elif not name and not (self.at_punct(",") or self.at_punct("]")):
             k = self.const_rational_expr()
+        # an empty slot (`a[,j+1]`) is the pattern "" + 0, as in Parser.hs:186-192
         if not (self.at_punct(",") or self.at_punct("]")):
             self.err("malformed grid index")
         return (name, k)

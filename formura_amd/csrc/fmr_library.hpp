@@ -175,6 +175,8 @@ class StepLibrary {
     FMR_CHECK(hipEventCreate(&evm));
     FMR_CHECK(hipEventCreate(&ev1));
     // default: the fused variant with T = 2 if compiled in, else the first one
+    // (programs without an inlined point function: the stage kernels, set by
+    // init_stages)
     choice = Variant{KIND_DIRECT, 1, 0, 1, 1, 1};
     for (int i = 0; i < nvariants; ++i)
       if (choice.kind == KIND_DIRECT || (variants[i].v.T == 2 && choice.T != 2)) choice = variants[i].v;
@@ -191,6 +193,7 @@ class StepLibrary {
     stages = st;
     nstages = ns;
     ntemps = nt;
+    if (!P::INLINED) choice = Variant{KIND_STAGED, 1, 0, 1, 1, 1};
     for (int t = 0; t < nt; ++t) {
       FMR_CHECK(hipMalloc(&temps[t], count * sizeof(real)));
       FMR_CHECK(hipMemsetAsync(temps[t], 0, count * sizeof(real), stream));
@@ -260,12 +263,17 @@ class StepLibrary {
   }
 
   void launch_direct(int steps) {
+    if constexpr (!P::INLINED) {
+      fprintf(stderr, "formura_amd: this program has no inlined point function (too large); use the stage kernels\n");
+      abort();
+    }
     const int threads = 256;
     const size_t blocks_needed = (count + threads - 1) / threads;
     const int blocks = (int)(blocks_needed < 65536 ? blocks_needed : 65536);
     for (int s = 0; s < steps; ++s) {
-      hipLaunchKernelGGL((direct_step_kernel<P>), dim3(blocks), dim3(threads), 0, stream, buf[cur], buf[1 - cur],
-                         n[0], n[1], n[2], prm);
+      if constexpr (P::INLINED)
+        hipLaunchKernelGGL((direct_step_kernel<P>), dim3(blocks), dim3(threads), 0, stream, buf[cur], buf[1 - cur],
+                           n[0], n[1], n[2], prm);
       FMR_CHECK(hipGetLastError());
       cur = 1 - cur;
     }

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // the reference's loop nest per manifest node does, instead of being
 // recomputed inside its users.
 // ===========================================================================
-constexpr int MAX_ARRAYS = 64;
+constexpr int MAX_ARRAYS = 128;   // state fields + stage arrays of one program
 
 template <class real>
 struct Arrays {

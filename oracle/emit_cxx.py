@@ -65,6 +65,7 @@ def emit_cxx(lay: Layout, mm: MMProgram, params=()) -> str:
     w = out.append
     w("/* ORACLE: reference-faithful CPU step library (generated). */")
     w("#include <math.h>")
+    w("#include <stdbool.h>")
     w("#include <string.h>")
     w(f'#include "{lay.name}.h"')
     w("")
