@@ -11,7 +11,10 @@ short timed search before the timed region; the best rate for each compiled
 fusion depth T is reported as well (t_sweep_glups).
 
 Also reported (same line, key "extra"): the FP32 bulk heat rate, 3D Gray-Scott
-512^3 in FP64 and FP32, 3D Gray-Scott 64^3 FP64 (1000 steps), the batched 2D
+512^3 in FP64 and FP32, 3D Gray-Scott 64^3 FP64 (1000 steps), the multi-stage
+programs (explicit midpoint heat 256^3, RK4 Gray-Scott 64^3, finite-volume
+isothermal gas dynamics with SSP-RK3 256^3: stage kernels, with their own
+roofline, and the inlined point function where there is one), the batched 2D
 Gray-Scott sweep (256 uneven instances x 5000 steps, FP32 vs FP64 vs the plain
 method) and the checked surrogate on held-out 256^2 trajectories.
 
@@ -39,18 +42,48 @@ def log(msg):
 
 def seeded_state(lib, seed):
     import numpy as np
+    from formura_amd.rng import gray_scott_ic, normal
     rng = np.random.default_rng(seed)
     shape = tuple(lib.layout.intra)
     if lib.layout.field_names == ["u"]:
         return {"u": rng.standard_normal(shape)}
-    from tests.common import gray_scott_ic
+    if lib.layout.field_names == ["r", "mx", "my", "mz"]:
+        r = 1.0 + 0.2 * normal(seed, shape)
+        return {"r": r, "mx": r * 0.1 * normal(seed + 1, shape), "my": r * 0.1 * normal(seed + 2, shape),
+                "mz": r * 0.1 * normal(seed + 3, shape)}
     u, v = gray_scott_ic(shape, seed)
     return {"U": u, "V": v}
 
 
+def has_direct(lib):
+    """Whether the library has the inlined point function (direct kernel)."""
+    from formura_amd.runtime import KIND_DIRECT
+    try:
+        lib.set_kernel(KIND_DIRECT)
+        return True
+    except ValueError:
+        return False
+
+
+def staged_bytes_per_site(name):
+    """Algorithmic bytes per site and step of the stage kernels: each kernel
+    reads each array it loads once and writes each of its outputs once."""
+    from formura_amd import build
+    from formura_amd.compiler.driver import compile_files
+    from formura_amd.compiler.pointir import group_stages, lower_stages
+    fmr, yaml_, precision, params, over, _ = build.PROGRAMS[name]
+    c = compile_files(name, fmr, yaml_, precision, params, over)
+    elem = 8 if precision == "faithful" else 4
+    total = 0
+    for g in group_stages(lower_stages(c.mm, precision, params)):
+        arrays = {op.args[0] for op in g.ir.ops if op.kind == "load"}
+        total += (len(arrays) + len(g.members)) * elem
+    return total
+
+
 def candidates(lib):
     from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED
-    out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)]
+    out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)] if has_direct(lib) else []
     if lib.num_stages() > 0:
         out.append(((KIND_STAGED, 1, 0, 1, 1, 1), 64))
     for v in lib.list_kernels():
@@ -130,6 +163,52 @@ def cpu_baseline(sample_steps=20):
                       f"-ffp-contract=off), {dt:.1f} s"}
 
 
+# registered program -> label of its line under "extra"
+EXTRA_PROGRAMS = (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp64"),
+                  ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64"),
+                  ("heat_rk2", "heat256_midpoint_fp64"), ("gs3d_rk4", "grayscott64_rk4_fp64"),
+                  ("fveuler", "fv_euler256_ssprk3_fp64"))
+
+
+def program_extras(only=None):
+    """Autotuned rate of each program of EXTRA_PROGRAMS (or only the one
+    labelled `only`); multi-stage programs also report their stage kernels."""
+    import numpy as np
+    from formura_amd.runtime import load_program
+    extra = {}
+    for name, label in EXTRA_PROGRAMS:
+        if only and label != only:
+            continue
+        lg = load_program(name)
+        lg.init()
+        lg.set_state(seeded_state(lg, 1))
+        lg.upload()
+        n = int(np.prod(lg.layout.intra))
+        big = n >= 2 ** 24 or lg.num_stages() > 0
+        best = autotune(lg, probe_steps=24 if big else 200, final_steps=96 if big else 1000)
+        r, v, l = max(best.values())
+        steps = 100 if big else 1000
+        w, _, _, _ = timed_run(lg, v, l, steps, 10)
+        extra[label] = {"glups": round(steps * n / w / 1e9, 2), "kernel": list(v) + [l],
+                        "best_per_T": {T: round(x[0], 1) for T, x in sorted(best.items())}}
+        if lg.num_stages() > 0:   # multi-stage program: stage kernels vs the inlined point function
+            from formura_amd.runtime import KIND_DIRECT, KIND_STAGED
+            kinds = [(KIND_STAGED, "staged_glups")] + ([(KIND_DIRECT, "inlined_direct_glups")]
+                                                       if has_direct(lg) else [])
+            for kind, key in kinds:
+                w2, ev2, _, _ = timed_run(lg, (kind, 1, 0, 1, 1, 1), 64, 20, 2)
+                extra[label][key] = round(20 * n / w2 / 1e9, 2)
+                if kind == KIND_STAGED:
+                    b = staged_bytes_per_site(name) * n
+                    gbs = b * 20 / (ev2 * 1e-3) / 1e9
+                    extra[label]["staged"] = {
+                        "manifest_nodes": lg.num_stages(), "kernels_per_step": lg.num_stage_kernels(),
+                        "algorithmic_bytes_per_step": b, "ms_per_step": round(ev2 / 20, 4),
+                        "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"{label}: {extra[label]}")
+    return extra
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +217,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,occ,lx: skip the search")
+    ap.add_argument("--only", default="", help="run only this line of 'extra' (e.g. for a profile)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
               "not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if a.only:
+        print(json.dumps({"only": a.only, "extra": program_extras(a.only)}), flush=True)
+        return
 
     import numpy as np
     from formura_amd.runtime import load_program
@@ -212,27 +296,7 @@ def main():
 
     if not a.no_extra:
         extra = {}
-        for name, label in (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp64"),
-                            ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64"),
-                            ("heat_rk2", "heat256_midpoint_fp64"), ("gs3d_rk4", "grayscott64_rk4_fp64")):
-            lg = load_program(name)
-            lg.init()
-            lg.set_state(seeded_state(lg, 1))
-            lg.upload()
-            big = "512" in name or "rk" in name
-            best = autotune(lg, probe_steps=24 if big else 200, final_steps=96 if big else 1000)
-            r, v, l = max(best.values())
-            steps = 100 if "512" in name or "rk" in name else 1000
-            w, _, _, _ = timed_run(lg, v, l, steps, 10)
-            n = int(np.prod(lg.layout.intra))
-            extra[label] = {"glups": round(steps * n / w / 1e9, 2), "kernel": list(v) + [l],
-                            "best_per_T": {T: round(x[0], 1) for T, x in sorted(best.items())}}
-            if lg.num_stages() > 0:   # multi-stage program: stage kernels vs the inlined point function
-                from formura_amd.runtime import KIND_DIRECT, KIND_STAGED
-                for kind, key in ((KIND_STAGED, "staged_glups"), (KIND_DIRECT, "inlined_direct_glups")):
-                    w2, _, _, _ = timed_run(lg, (kind, 1, 0, 1, 1, 1), 64, 20, 2)
-                    extra[label][key] = round(20 * n / w2 / 1e9, 2)
-            log(f"{label}: {extra[label]}")
+        extra = program_extras()
         out["extra"] = extra
         # batched Gray-Scott sweep: 256 uneven 2D instances x 5000 steps, FP32
         # bulk (hipGraph, autotuned plan) vs its FP64 twin and the plain method

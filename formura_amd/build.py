@@ -60,10 +60,10 @@ PROGRAMS: Dict[str, tuple] = {
     "gs3d_rk4": (_ex("gray-scott-3d-rk4.fmr"), _ex("gray-scott-3d-rk4.yaml"), "faithful", (), None, None),
     "gs3d_rk4_t": (_ex("gray-scott-3d-rk4.fmr"), _ex("gray-scott-3d-rk4.yaml"), "faithful", (),
                    {"intra_node_shape": [20, 34, 40], "monitor_interval": 4}, None),
-    # isothermal Euler, RK4 + minmod-limited viscosity: 40 manifest nodes, a point
-    # function too large to inline (stage kernels only)
-    "euler3d": (_ex("euler-3d-rk4.fmr"), _ex("euler-3d-rk4.yaml"), "faithful", (), None, None),
-    "euler3d_t": (_ex("euler-3d-rk4.fmr"), _ex("euler-3d-rk4.yaml"), "faithful", (),
+    # isothermal gas dynamics, finite volumes (MUSCL + Rusanov), SSP-RK3: 48
+    # manifest nodes, a point function too large to inline (stage kernels only)
+    "fveuler": (_ex("fv-euler-3d.fmr"), _ex("fv-euler-3d.yaml"), "faithful", (), None, None),
+    "fveuler_t": (_ex("fv-euler-3d.fmr"), _ex("fv-euler-3d.yaml"), "faithful", (),
                   {"intra_node_shape": [22, 26, 34], "monitor_interval": 2}, None),
 }
 

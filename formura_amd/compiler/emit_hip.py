@@ -20,7 +20,7 @@ import heapq
 from typing import Dict, List, Sequence, Tuple
 
 from .abi import Layout, init_body
-from .pointir import PointIR, Stage, c_expr_of, imm_literal
+from .pointir import PointIR, Stage, c_expr_of, group_stages, imm_literal
 from .shiftreuse import ShiftAnalysis, depends_on, op_children
 
 # Registers (32-bit, per lane) of one march wavefront, as the compiler
@@ -214,22 +214,27 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("};")
     w("")
     ntemps = sum(1 for st in stages if st.target == "temp")
-    for k, st in enumerate(stages):
-        w(f"// manifest node {st.om_id} -> {'state field ' + lay.field_names[st.index] if st.target == 'state' else 'stage array ' + str(st.index)}")
+    groups = group_stages(stages) if stages else []
+    for k, g in enumerate(groups):
+        for st in g.members:
+            dest = "state field " + lay.field_names[st.index] if st.target == "state" else "stage array " + str(st.index)
+            w(f"// manifest node {st.om_id} -> {dest}")
         w(f"struct Stage{k} {{")
         w(f"  using real = {real};")
         w(f"  static constexpr int DIM = {dim};")
-        w("  static constexpr int NF = 1;")
+        w(f"  static constexpr int NOUT = {len(g.members)};")
         w("  template <class A>")
         w("  __device__ __forceinline__ static void point(A& a) {")
-        w("    " + point_body(st.ir, real))
+        w("    " + point_body(g.ir, real))
         w("  }")
         w("};")
         w("")
-    if stages:
+    if groups:
         w("const fmr::StageEntry<Prog> kStages[] = {")
-        for k, st in enumerate(stages):
-            w(f"  {{&fmr::launch_stage<Prog, Stage{k}>, {1 if st.target == 'state' else 0}, {st.index}}},")
+        for k, g in enumerate(groups):
+            ts = ", ".join("1" if st.target == "state" else "0" for st in g.members)
+            ix = ", ".join(str(st.index) for st in g.members)
+            w(f"  {{&fmr::launch_stage<Prog, Stage{k}>, {len(g.members)}, {{{ts}}}, {{{ix}}}}},")
         w("};")
         w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
@@ -257,7 +262,7 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  " + init_body(lay))
     w(f"  lib.init({n[0]}, {n[1]}, {n[2]}, kVariants, kNumVariants);")
     if stages:
-        w(f"  lib.init_stages(kStages, {len(stages)}, {ntemps});")
+        w(f"  lib.init_stages(kStages, {len(groups)}, {ntemps});")
     w("  return 0;")
     w("}")
     w("")
@@ -330,6 +335,11 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("int Formura_Num_Stages (void)")
     w("{")
     w("  return lib.nstages;")
+    w("}")
+    w("")
+    w("int Formura_Num_Stage_Kernels (void)")
+    w("{")
+    w("  return lib.nkernels;")
     w("}")
     w("")
     w("int Formura_Set_March_Chunk (int lx)")

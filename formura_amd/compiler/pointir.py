@@ -248,6 +248,95 @@ def lower_stages(mm, precision: str = "faithful", params: Sequence[str] = ()) ->
     return out
 
 
+@dataclass
+class StageGroup:
+    """Manifest nodes evaluated by one kernel: none of them reads an array
+    another member writes, so they can run over the grid together.  `ir` has
+    one output per member, in member order; equal loads and equal operations
+    of different members are computed once (the same IEEE operations on the
+    same operands: the same bits as evaluating each member alone)."""
+    ir: PointIR
+    members: List[Stage]
+
+
+def _op_key(op: Op, remap: Dict[int, int]) -> tuple:
+    k, a = op.kind, op.args
+    if k == "uniop":
+        args = (a[0], remap[a[1]])
+    elif k == "binop":
+        args = (a[0], remap[a[1]], remap[a[2]])
+    elif k == "triop":
+        args = (a[0],) + tuple(remap[x] for x in a[1:4])
+    elif k == "naryop":
+        args = (a[0], tuple(remap[x] for x in a[1]))
+    elif k == "load":
+        args = (a[0], tuple(a[1]))
+    else:
+        args = tuple(a)
+    return (k, args, op.ctype)
+
+
+def _merge(members: Sequence[Stage]) -> PointIR:
+    ops: List[Op] = []
+    seen: Dict[tuple, int] = {}
+    outputs: List[int] = []
+    for st in members:
+        remap: Dict[int, int] = {}
+        for i, op in enumerate(st.ir.ops):
+            key = _op_key(op, remap)
+            j = seen.get(key)
+            if j is None:
+                k, a = key[0], key[1]
+                if k == "naryop":
+                    args = (a[0], list(a[1]))
+                elif k == "load":
+                    args = (a[0], a[1])
+                else:
+                    args = a
+                j = len(ops)
+                ops.append(Op(k, args, op.ctype))
+                seen[key] = j
+            remap[i] = j
+        outputs.append(remap[st.ir.outputs[0]])
+    ir0 = members[0].ir
+    return PointIR(ir0.dim, ir0.axes, ir0.extents, ir0.fields, ir0.field_ctypes, ops, outputs, ir0.params)
+
+
+def group_stages(stages: Sequence[Stage], max_loads: int = 64, max_ops: int = 1200,
+                 max_members: int = 16) -> List[StageGroup]:
+    """Independent manifest nodes side by side: a node's level is one more
+    than the highest level of the stage arrays it reads (0 when it reads only
+    state fields); nodes of one level never read each other's arrays and
+    become one kernel, split where a group would exceed `max_loads` distinct
+    loads, `max_ops` operations or `max_members` outputs.  New state values
+    go to the other buffer of the double-buffered state, so a node writing
+    one never conflicts with a node reading the current state."""
+    nf = len(stages[0].ir.fields) - sum(1 for s in stages if s.target == "temp") if stages else 0
+    level_of_array: Dict[int, int] = {}
+    levels: Dict[int, List[Stage]] = {}
+    for st in stages:
+        reads = {op.args[0] for op in st.ir.ops if op.kind == "load"}
+        lv = 1 + max((level_of_array[a] for a in reads if a >= nf), default=-1)
+        if st.target == "temp":
+            level_of_array[st.index] = lv
+        levels.setdefault(lv, []).append(st)
+    out: List[StageGroup] = []
+    for lv in sorted(levels):
+        cur: List[Stage] = []
+        for st in levels[lv]:
+            trial = cur + [st]
+            ir = _merge(trial)
+            nloads = sum(1 for op in ir.ops if op.kind == "load")
+            if cur and (nloads > max_loads or len(ir.ops) > max_ops or len(trial) > max_members):
+                out.append(StageGroup(_merge(cur), cur))
+                cur = [st]
+            else:
+                cur = trial
+        if cur:
+            out.append(StageGroup(_merge(cur), cur))
+    return out
+
+
 def imm_literal(r: Fraction, ctype: str) -> str:
     """C literal of a rational: the nearest double (MPICxx/Translate.hs:585,
     `realToFrac r :: Double` is correctly rounded), printed shortest

@@ -101,18 +101,26 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
 // one manifest node of a multi-stage step (stage_kernel<S>)
 template <class P>
 struct StageEntry {
-  void (*fn)(const Arrays<typename P::real>&, typename P::real*, int, int, int, const Params&, hipStream_t);
-  int to_state;   // 1: writes the new value of state field `index`; 0: stage array `index`
-  int index;      // in the array list: state fields first, then stage arrays
+  void (*fn)(const Arrays<typename P::real>&, const Outs<typename P::real>&, int, int, int, const Params&,
+             hipStream_t);
+  int nout;                 // manifest nodes evaluated by this kernel
+  int to_state[MAX_OUTS];   // 1: writes the new value of state field `index`; 0: stage array `index`
+  int index[MAX_OUTS];      // in the array list: state fields first, then stage arrays
 };
 
 template <class P, class S>
-void launch_stage(const Arrays<typename P::real>& in, typename P::real* out, int n0, int n1, int n2,
+void launch_stage(const Arrays<typename P::real>& in, const Outs<typename P::real>& out, int n0, int n1, int n2,
                   const Params& prm, hipStream_t s) {
-  const size_t count = (size_t)n0 * n1 * n2;
-  const size_t need = (count + 255) / 256;
-  const int blocks = (int)(need < 65536 ? need : 65536);
-  hipLaunchKernelGGL((stage_kernel<S>), dim3(blocks), dim3(256), 0, s, in, out, n0, n1, n2, prm);
+  constexpr int LANE = P::DIM - 1;
+  const int n[3] = {n0, n1, n2};
+  const int row = LANE == 2 ? 1 : 0, slab = LANE == 2 ? 0 : 2;
+  const int gy = (n[row] + STAGE_ROWS - 1) / STAGE_ROWS;
+  if (gy > 65535 || n[slab] > 65535) {
+    fprintf(stderr, "formura_amd: grid %d x %d x %d too large for the stage kernels\n", n0, n1, n2);
+    abort();
+  }
+  const dim3 grid((n[LANE] + 63) / 64, gy, n[slab]);
+  hipLaunchKernelGGL((stage_kernel<S, LANE>), grid, dim3(64, STAGE_ROWS), 0, s, in, out, n0, n1, n2, prm);
 }
 
 template <class P>
@@ -144,7 +152,8 @@ class StepLibrary {
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   // multi-stage programs: the manifest-node kernels and their arrays
   const StageEntry<P>* stages = nullptr;
-  int nstages = 0;
+  int nkernels = 0;   // stage kernels per step
+  int nstages = 0;    // manifest nodes they evaluate
   int ntemps = 0;
   real* temps[MAX_ARRAYS] = {};
   bool ready = false;
@@ -184,14 +193,16 @@ class StepLibrary {
     ready = true;
   }
 
-  // the manifest-node kernels of a multi-stage step, with ntemp stage arrays
-  void init_stages(const StageEntry<P>* st, int ns, int nt) {
+  // the nk stage kernels of a multi-stage step, with nt stage arrays
+  void init_stages(const StageEntry<P>* st, int nk, int nt) {
     if (P::NF + nt > MAX_ARRAYS) {
       fprintf(stderr, "formura_amd: %d state fields + %d stage arrays exceed %d\n", P::NF, nt, MAX_ARRAYS);
       abort();
     }
     stages = st;
-    nstages = ns;
+    nkernels = nk;
+    nstages = 0;
+    for (int k = 0; k < nk; ++k) nstages += st[k].nout;
     ntemps = nt;
     if (!P::INLINED) choice = Variant{KIND_STAGED, 1, 0, 1, 1, 1};
     for (int t = 0; t < nt; ++t) {
@@ -208,10 +219,13 @@ class StepLibrary {
       for (int i = 0; i < MAX_ARRAYS; ++i) a.a[i] = nullptr;
       for (int f = 0; f < P::NF; ++f) a.a[f] = buf[cur].f[f];
       for (int t = 0; t < ntemps; ++t) a.a[P::NF + t] = temps[t];
-      for (int k = 0; k < nstages; ++k) {
+      for (int k = 0; k < nkernels; ++k) {
         const StageEntry<P>& e = stages[k];
-        real* out = e.to_state ? buf[1 - cur].f[e.index] : temps[e.index - P::NF];
-        e.fn(a, out, n[0], n[1], n[2], prm, stream);
+        Outs<real> o;
+        for (int j = 0; j < MAX_OUTS; ++j) o.o[j] = nullptr;
+        for (int j = 0; j < e.nout; ++j)
+          o.o[j] = e.to_state[j] ? buf[1 - cur].f[e.index[j]] : temps[e.index[j] - P::NF];
+        e.fn(a, o, n[0], n[1], n[2], prm, stream);
         FMR_CHECK(hipGetLastError());
       }
       cur = 1 - cur;

@@ -161,20 +161,31 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 }
 
 // ===========================================================================
-// Kernel 1b: one manifest node of a multi-stage step (a program with
-// `manifest ::` declarations, e.g. Runge-Kutta stages), one site per thread.
-// `S::point` reads the state fields and the arrays of earlier manifest nodes
-// through `a.template ld<ARRAY, D0, D1, D2>()` (ARRAY indexes in.a: state
-// fields first, then the stage arrays) and writes one value with
-// `a.template st<0>(v)`.  Each node is evaluated once per site and stored, as
+// Kernel 1b: manifest nodes of a multi-stage step (a program with
+// `manifest ::` declarations, e.g. Runge-Kutta stages).  One kernel evaluates
+// a group of nodes that do not read each other's arrays; `S::point` reads the
+// state fields and the arrays of earlier nodes through
+// `a.template ld<ARRAY, D0, D1, D2>()` (ARRAY indexes in.a: state fields
+// first, then the stage arrays) and writes member k's value with
+// `a.template st<k>(v)`.  Each node is evaluated once per site and stored, as
 // the reference's loop nest per manifest node does, instead of being
 // recomputed inside its users.
+//
+// Thread mapping: the contiguous axis LANE (z in 3D, y in 2D) across the 64
+// lanes of a wavefront, 4 wavefronts on 4 consecutive positions of the next
+// axis, the remaining axis over grid.z -- no index division, coalesced rows.
 // ===========================================================================
 constexpr int MAX_ARRAYS = 128;   // state fields + stage arrays of one program
+constexpr int MAX_OUTS = 16;      // manifest nodes per stage kernel
 
 template <class real>
 struct Arrays {
   real* a[MAX_ARRAYS];
+};
+
+template <class real>
+struct Outs {
+  real* o[MAX_OUTS];
 };
 
 template <class S>
@@ -184,7 +195,7 @@ struct StageAcc : EvalReuse {
   const Params& prm;
   int c[3];
   int n[3];
-  real out[1];
+  real out[S::NOUT];
 
   template <int F, int D0, int D1, int D2>
   __device__ __forceinline__ real ld() const {
@@ -199,16 +210,25 @@ struct StageAcc : EvalReuse {
   __device__ __forceinline__ void st(real v) { out[F] = v; }
 };
 
-template <class S>
-__global__ void __launch_bounds__(256) stage_kernel(Arrays<typename S::real> in, typename S::real* out, int n0,
-                                                    int n1, int n2, Params prm) {
-  const size_t total = (size_t)n0 * n1 * n2;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    StageAcc<S> a{{}, in, prm, {(int)(i / ((size_t)n1 * n2)), (int)((i / n2) % n1), (int)(i % n2)}, {n0, n1, n2}, {}};
-    S::point(a);
-    out[i] = a.out[0];
-  }
+constexpr int STAGE_ROWS = 4;     // wavefronts per workgroup, one row each
+
+template <class S, int LANE>
+__global__ void __launch_bounds__(64 * STAGE_ROWS) stage_kernel(Arrays<typename S::real> in,
+                                                               Outs<typename S::real> out, int n0, int n1,
+                                                               int n2, Params prm) {
+  constexpr int ROW = LANE == 2 ? 1 : 0;    // axis of the 4 rows
+  constexpr int SLAB = LANE == 2 ? 0 : 2;   // axis over grid.z (extent 1 in 2D)
+  const int n[3] = {n0, n1, n2};
+  int c[3];
+  c[LANE] = blockIdx.x * 64 + threadIdx.x;
+  c[ROW] = blockIdx.y * STAGE_ROWS + threadIdx.y;
+  c[SLAB] = blockIdx.z;
+  if (c[LANE] >= n[LANE] || c[ROW] >= n[ROW]) return;
+  StageAcc<S> a{{}, in, prm, {c[0], c[1], c[2]}, {n0, n1, n2}, {}};
+  S::point(a);
+  const size_t i = ((size_t)c[0] * n1 + c[1]) * n2 + c[2];
+#pragma unroll
+  for (int k = 0; k < S::NOUT; ++k) out.o[k][i] = a.out[k];
 }
 
 // ===========================================================================

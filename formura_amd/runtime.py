@@ -66,6 +66,7 @@ class StepLibrary:
             L.Formura_Get_Kernel.argtypes = [ctypes.POINTER(ctypes.c_int)] * 7
             L.Formura_List_Kernels.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
             L.Formura_Num_Stages.restype = ctypes.c_int
+            L.Formura_Num_Stage_Kernels.restype = ctypes.c_int
             L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.Formura_Device_Field.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.restype = ctypes.c_void_p
@@ -113,8 +114,13 @@ class StepLibrary:
         return float(self.lib.Formura_Forward_Device_Timed(ctypes.byref(self.navi), int(nsteps)))
 
     def num_stages(self) -> int:
-        """Manifest-node kernels of a multi-stage program (0: single stage)."""
+        """Manifest nodes of a multi-stage program evaluated by stage kernels
+        (0: single stage)."""
         return int(self.lib.Formura_Num_Stages())
+
+    def num_stage_kernels(self) -> int:
+        """Stage kernels launched per step (independent nodes share one)."""
+        return int(self.lib.Formura_Num_Stage_Kernels())
 
     def last_timing(self):
         """(ms of the selected kernel's launches, their count, total ms) of the

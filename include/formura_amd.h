@@ -67,9 +67,13 @@ int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
  * Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
 
-/* Number of manifest-node kernels (kind 2) compiled into the library; 0 for a
- * program whose only manifest nodes are the state updates. */
+/* Number of manifest nodes the stage kernels (kind 2) evaluate per step; 0
+ * for a program whose only manifest nodes are the state updates. */
 int Formura_Num_Stages(void);
+
+/* Number of stage kernels launched per step: manifest nodes that do not read
+ * each other's arrays share one kernel (at most 16 per kernel). */
+int Formura_Num_Stage_Kernels(void);
 
 /* Current kernel selection. */
 int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *occ, int *lx);

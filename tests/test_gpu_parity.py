@@ -196,28 +196,30 @@ def test_multistage_programs_bitwise(name, mk):
         assert_bitwise(got, want, f"{name} kind={kind}")
 
 
-def euler_state(lib, seed):
+def fv_state(lib, seed):
     n = tuple(lib.layout.intra)
-    return {"rho": 1.0 + 0.05 * normal(seed, n), "vx": 0.05 * normal(seed + 1, n),
-            "vy": 0.05 * normal(seed + 2, n), "vz": 0.05 * normal(seed + 3, n)}
+    r = 1.0 + 0.2 * normal(seed, n)
+    return {"r": r, "mx": r * 0.1 * normal(seed + 1, n), "my": r * 0.1 * normal(seed + 2, n),
+            "mz": r * 0.1 * normal(seed + 3, n)}
 
 
 def test_large_multistage_program_bitwise():
-    """Isothermal Euler with four Runge-Kutta stages and limited face
-    viscosity: 40 manifest nodes, a point function of ~146k ops that is not
-    inlined, so only the stage kernels are compiled; they reproduce the oracle
-    bit for bit, and the direct kernel is refused."""
-    lib = hip("euler3d_t")
-    assert lib.num_stages() == 40
+    """Isothermal gas dynamics, finite volumes (MUSCL + Rusanov) with SSP-RK3:
+    48 manifest nodes, a point function of ~39k ops that is not inlined, so
+    only the stage kernels are compiled (6 per step, independent nodes side
+    by side); they reproduce the oracle bit for bit, and the direct kernel is
+    refused."""
+    lib = hip("fveuler_t")
+    assert lib.num_stages() == 48 and lib.num_stage_kernels() == 6
     with pytest.raises(ValueError):
         lib.set_kernel(KIND_DIRECT)
     lib.set_kernel(KIND_STAGED)
     steps = lib.layout.steps_per_forward
-    st = euler_state(lib, 17)
-    want = oracle_run("euler3d_t", st, 3)
-    _say(f"[parity] euler3d_t kind=staged stages=40 steps={3 * steps}")
+    st = fv_state(lib, 17)
+    want = oracle_run("fveuler_t", st, 3)
+    _say(f"[parity] fveuler_t kind=staged stages=48 steps={3 * steps}")
     got = device_run(lib, st, 3 * steps)
-    assert_bitwise(got, want, "euler3d_t staged")
+    assert_bitwise(got, want, "fveuler_t staged")
     lib.set_state(st)
     lib.forward()
-    assert_bitwise(lib.get_state(), oracle_run("euler3d_t", st, 1), "euler3d_t Formura_Forward")
+    assert_bitwise(lib.get_state(), oracle_run("fveuler_t", st, 1), "fveuler_t Formura_Forward")
