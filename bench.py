@@ -85,7 +85,8 @@ def candidates(lib):
     from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED
     out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)] if has_direct(lib) else []
     if lib.num_stages() > 0:
-        out.append(((KIND_STAGED, 1, 0, 1, 1, 1), 64))
+        for rows in (4, 16):     # wavefronts per workgroup of the stage kernels
+            out.append(((KIND_STAGED, 1, rows, 1, 1, 1), 64))
     for v in lib.list_kernels():
         for lx in (32, 64, 128, -1, -2):   # run length, or -workgroups per SIMD
             out.append(((KIND_MARCH,) + tuple(v), lx))
@@ -196,13 +197,18 @@ def program_extras(only=None):
             kinds = [(KIND_STAGED, "staged_glups")] + ([(KIND_DIRECT, "inlined_direct_glups")]
                                                        if has_direct(lg) else [])
             for kind, key in kinds:
-                w2, ev2, _, _ = timed_run(lg, (kind, 1, 0, 1, 1, 1), 64, 20, 2)
+                # stage kernels: workgroups of 4 or 16 wavefronts, the faster one
+                runs = {rows: timed_run(lg, (kind, 1, rows, 1, 1, 1), 64, 20, 2)
+                        for rows in ((4, 16) if kind == KIND_STAGED else (0,))}
+                rows = min(runs, key=lambda r: runs[r][0])
+                w2, ev2, _, _ = runs[rows]
                 extra[label][key] = round(20 * n / w2 / 1e9, 2)
                 if kind == KIND_STAGED:
                     b = staged_bytes_per_site(name) * n
                     gbs = b * 20 / (ev2 * 1e-3) / 1e9
                     extra[label]["staged"] = {
                         "manifest_nodes": lg.num_stages(), "kernels_per_step": lg.num_stage_kernels(),
+                        "rows": rows, "glups_by_rows": {r: round(20 * n / v[0] / 1e9, 2) for r, v in runs.items()},
                         "algorithmic_bytes_per_step": b, "ms_per_step": round(ev2 / 20, 4),
                         "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
         log(f"{label}: {extra[label]}")

@@ -323,8 +323,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("    return 0;")
     w("  }")
     w("  if (kind == fmr::KIND_STAGED) {")
-    w("    if (lib.nstages == 0) return -1;")
-    w("    lib.choice = fmr::Variant{fmr::KIND_STAGED, 1, 0, 1, 1, 1};")
+    w("    if (lib.nstages == 0 || (rows != 0 && rows != 4 && rows != 16)) return -1;")
+    w("    lib.choice = fmr::Variant{fmr::KIND_STAGED, 1, rows, 1, 1, 1};")
     w("    return 0;")
     w("  }")
     w("  if (!lib.find(kind, T, rows, waves, pf, occ)) return -1;")

@@ -101,26 +101,45 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
 // one manifest node of a multi-stage step (stage_kernel<S>)
 template <class P>
 struct StageEntry {
-  void (*fn)(const Arrays<typename P::real>&, const Outs<typename P::real>&, int, int, int, const Params&,
+  void (*fn)(const Arrays<typename P::real>&, const Outs<typename P::real>&, int, int, int, const Params&, int,
              hipStream_t);
   int nout;                 // manifest nodes evaluated by this kernel
   int to_state[MAX_OUTS];   // 1: writes the new value of state field `index`; 0: stage array `index`
   int index[MAX_OUTS];      // in the array list: state fields first, then stage arrays
 };
 
-template <class P, class S>
-void launch_stage(const Arrays<typename P::real>& in, const Outs<typename P::real>& out, int n0, int n1, int n2,
-                  const Params& prm, hipStream_t s) {
+template <class P, class S, int ROWS>
+void launch_stage_rows(const Arrays<typename P::real>& in, const Outs<typename P::real>& out, int n0, int n1, int n2,
+                       const Params& prm, hipStream_t s) {
   constexpr int LANE = P::DIM - 1;
   const int n[3] = {n0, n1, n2};
   const int row = LANE == 2 ? 1 : 0, slab = LANE == 2 ? 0 : 2;
-  const int gy = (n[row] + STAGE_ROWS - 1) / STAGE_ROWS;
+  const int gy = (n[row] + ROWS - 1) / ROWS;
   if (gy > 65535 || n[slab] > 65535) {
     fprintf(stderr, "formura_amd: grid %d x %d x %d too large for the stage kernels\n", n0, n1, n2);
     abort();
   }
   const dim3 grid((n[LANE] + 63) / 64, gy, n[slab]);
-  hipLaunchKernelGGL((stage_kernel<S, LANE>), grid, dim3(64, STAGE_ROWS), 0, s, in, out, n0, n1, n2, prm);
+  for (int a = 0; a < 3; ++a)
+    if (n[a] < (a == 0 ? P::RX : a == 1 ? P::RY : P::RZ)) {
+      fprintf(stderr, "formura_amd: grid %d x %d x %d is narrower than the stencil reach\n", n0, n1, n2);
+      abort();
+    }
+  const bool wide = (size_t)n0 * n1 * n2 * sizeof(typename P::real) > 0xffffffffull;
+  if (wide)
+    hipLaunchKernelGGL((stage_kernel<S, LANE, true, ROWS>), grid, dim3(64, ROWS), 0, s, in, out, n0, n1, n2, prm);
+  else
+    hipLaunchKernelGGL((stage_kernel<S, LANE, false, ROWS>), grid, dim3(64, ROWS), 0, s, in, out, n0, n1, n2, prm);
+}
+
+// rows: wavefronts per workgroup, 4 (default, also for 0) or 16
+template <class P, class S>
+void launch_stage(const Arrays<typename P::real>& in, const Outs<typename P::real>& out, int n0, int n1, int n2,
+                  const Params& prm, int rows, hipStream_t s) {
+  if (rows == 16)
+    launch_stage_rows<P, S, 16>(in, out, n0, n1, n2, prm, s);
+  else
+    launch_stage_rows<P, S, 4>(in, out, n0, n1, n2, prm, s);
 }
 
 template <class P>
@@ -225,7 +244,7 @@ class StepLibrary {
         for (int j = 0; j < MAX_OUTS; ++j) o.o[j] = nullptr;
         for (int j = 0; j < e.nout; ++j)
           o.o[j] = e.to_state[j] ? buf[1 - cur].f[e.index[j]] : temps[e.index[j] - P::NF];
-        e.fn(a, o, n[0], n[1], n[2], prm, stream);
+        e.fn(a, o, n[0], n[1], n[2], prm, choice.by, stream);
         FMR_CHECK(hipGetLastError());
       }
       cur = 1 - cur;

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #define FMR_CHECK(expr)                                                              \
   do {                                                                               \
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(256) direct_step_kernel(Fields<typename P::rea
 // recomputed inside its users.
 //
 // Thread mapping: the contiguous axis LANE (z in 3D, y in 2D) across the 64
-// lanes of a wavefront, 4 wavefronts on 4 consecutive positions of the next
+// lanes of a wavefront, ROWS wavefronts on consecutive positions of the next
 // axis, the remaining axis over grid.z -- no index division, coalesced rows.
 // ===========================================================================
 constexpr int MAX_ARRAYS = 128;   // state fields + stage arrays of one program
@@ -188,21 +189,42 @@ struct Outs {
   real* o[MAX_OUTS];
 };
 
-template <class S>
+// neighbour coordinate i + d with |d| <= n on a periodic axis of extent n
+__device__ __forceinline__ int wrap_near(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Addressing: the site's element offset `base`, plus one signed delta per
+// axis and neighbour distance (the same delta for every array, computed once
+// per distance: the compiler merges the identical expressions).  WIDE = false
+// (arrays below 4 GiB): 32-bit byte offsets added to the array's base in
+// scalar registers; WIDE = true: 64-bit element offsets.
+template <class S, bool WIDE>
 struct StageAcc : EvalReuse {
   using real = typename S::real;
+  using off_t = typename std::conditional<WIDE, long long, unsigned>::type;
   const Arrays<real>& in;
   const Params& prm;
   int c[3];
   int n[3];
+  off_t base;      // bytes (WIDE: elements) of the site from an array's start
+  off_t stride[3];
   real out[S::NOUT];
 
+  template <int A, int D>
+  __device__ __forceinline__ off_t delta() const {
+    if constexpr (D == 0) {
+      return 0;
+    } else {
+      return (off_t)(wrap_near(c[A] + D, n[A]) - c[A]) * stride[A];
+    }
+  }
   template <int F, int D0, int D1, int D2>
   __device__ __forceinline__ real ld() const {
-    const int x = (D0 == 0) ? c[0] : wrap(c[0] + D0, n[0]);
-    const int y = (D1 == 0) ? c[1] : wrap(c[1] + D1, n[1]);
-    const int z = (D2 == 0) ? c[2] : wrap(c[2] + D2, n[2]);
-    return in.a[F][((size_t)x * n[1] + y) * n[2] + z];
+    const off_t o = base + delta<0, D0>() + delta<1, D1>() + delta<2, D2>();
+    if constexpr (WIDE) {
+      return in.a[F][o];
+    } else {
+      return *(const real*)((const char*)in.a[F] + o);
+    }
   }
   __device__ __forceinline__ double param(int i) const { return prm.v[i]; }
   __device__ __forceinline__ int index(int axis) const { return c[axis]; }
@@ -210,21 +232,25 @@ struct StageAcc : EvalReuse {
   __device__ __forceinline__ void st(real v) { out[F] = v; }
 };
 
-constexpr int STAGE_ROWS = 4;     // wavefronts per workgroup, one row each
-
-template <class S, int LANE>
-__global__ void __launch_bounds__(64 * STAGE_ROWS) stage_kernel(Arrays<typename S::real> in,
-                                                               Outs<typename S::real> out, int n0, int n1,
-                                                               int n2, Params prm) {
-  constexpr int ROW = LANE == 2 ? 1 : 0;    // axis of the 4 rows
+// ROWS wavefronts per workgroup, one row each (4 or 16)
+template <class S, int LANE, bool WIDE, int ROWS>
+__global__ void __launch_bounds__(64 * ROWS) stage_kernel(Arrays<typename S::real> in, Outs<typename S::real> out,
+                                                         int n0, int n1, int n2, Params prm) {
+  using real = typename S::real;
+  using A = StageAcc<S, WIDE>;
+  using off_t = typename A::off_t;
+  constexpr int ROW = LANE == 2 ? 1 : 0;    // axis of the ROWS rows
   constexpr int SLAB = LANE == 2 ? 0 : 2;   // axis over grid.z (extent 1 in 2D)
   const int n[3] = {n0, n1, n2};
   int c[3];
   c[LANE] = blockIdx.x * 64 + threadIdx.x;
-  c[ROW] = blockIdx.y * STAGE_ROWS + threadIdx.y;
+  c[ROW] = blockIdx.y * ROWS + threadIdx.y;
   c[SLAB] = blockIdx.z;
   if (c[LANE] >= n[LANE] || c[ROW] >= n[ROW]) return;
-  StageAcc<S> a{{}, in, prm, {c[0], c[1], c[2]}, {n0, n1, n2}, {}};
+  const off_t unit = WIDE ? 1 : (off_t)sizeof(real);
+  const off_t st2 = unit, st1 = (off_t)n2 * st2, st0 = (off_t)n1 * st1;
+  const off_t base = (off_t)c[0] * st0 + (off_t)c[1] * st1 + (off_t)c[2] * st2;
+  A a{{}, in, prm, {c[0], c[1], c[2]}, {n0, n1, n2}, base, {st0, st1, st2}, {}};
   S::point(a);
   const size_t i = ((size_t)c[0] * n1 + c[1]) * n2 + c[2];
 #pragma unroll

@@ -53,9 +53,10 @@ float Formura_Forward_Device_Timed(struct Formura_Navigator *navi, int nsteps);
 int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
 
 /* Select the kernel: kind 0 = one site per thread (any stencil), kind 2 =
- * one kernel per manifest node of a multi-stage program (the reference's loop
- * nest per manifest node, MPICxx/Translate.hs:671-737; other arguments
- * ignored), kind 1 = temporally blocked march with T fused steps, `rows` y-rows per lane,
+ * the stage kernels of a multi-stage program (the reference's loop nest per
+ * manifest node, MPICxx/Translate.hs:671-737; `rows` = wavefronts per
+ * workgroup, 4 (also for 0) or 16; other arguments ignored),
+ * kind 1 = temporally blocked march with T fused steps, `rows` y-rows per lane,
  * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
  * ahead, compiled for at least `occ` waves per SIMD.  Returns 0, or -1 if that
  * variant is not compiled into the library. */

@@ -189,11 +189,11 @@ def test_multistage_programs_bitwise(name, mk):
     steps = lib.layout.steps_per_forward
     st = mk(lib, 13)
     want = oracle_run(name, st, 2)
-    for kind in (KIND_STAGED, KIND_DIRECT):
-        _say(f"[parity] {name} kind={kind} stages={lib.num_stages()} steps={2 * steps}")
-        lib.set_kernel(kind)
+    for kind, rows in ((KIND_STAGED, 4), (KIND_STAGED, 16), (KIND_DIRECT, 0)):
+        _say(f"[parity] {name} kind={kind} rows={rows} stages={lib.num_stages()} steps={2 * steps}")
+        lib.set_kernel(kind, 1, rows)
         got = device_run(lib, st, 2 * steps)
-        assert_bitwise(got, want, f"{name} kind={kind}")
+        assert_bitwise(got, want, f"{name} kind={kind} rows={rows}")
 
 
 def fv_state(lib, seed):
@@ -213,13 +213,16 @@ def test_large_multistage_program_bitwise():
     assert lib.num_stages() == 48 and lib.num_stage_kernels() == 6
     with pytest.raises(ValueError):
         lib.set_kernel(KIND_DIRECT)
-    lib.set_kernel(KIND_STAGED)
     steps = lib.layout.steps_per_forward
     st = fv_state(lib, 17)
     want = oracle_run("fveuler_t", st, 3)
-    _say(f"[parity] fveuler_t kind=staged stages=48 steps={3 * steps}")
-    got = device_run(lib, st, 3 * steps)
-    assert_bitwise(got, want, "fveuler_t staged")
+    for rows in (16, 4):
+        _say(f"[parity] fveuler_t kind=staged rows={rows} stages=48 steps={3 * steps}")
+        lib.set_kernel(KIND_STAGED, 1, rows)
+        got = device_run(lib, st, 3 * steps)
+        assert_bitwise(got, want, f"fveuler_t staged rows={rows}")
+    with pytest.raises(ValueError):
+        lib.set_kernel(KIND_STAGED, 1, 8)
     lib.set_state(st)
     lib.forward()
     assert_bitwise(lib.get_state(), oracle_run("fveuler_t", st, 1), "fveuler_t Formura_Forward")
