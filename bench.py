@@ -18,9 +18,11 @@ roofline, and the inlined point function where there is one), the batched 2D
 Gray-Scott sweep (256 uneven instances x 5000 steps, FP32 vs FP64 vs the plain
 method) and the checked surrogate on held-out 256^2 trajectories.
 
-cpu_baseline: the reference-faithful C/OpenMP oracle library of the same
-program, built with gcc -O3 -fopenmp -ffp-contract=off, on all host cores, on
-a bounded sample (20 steps of the full 512^3 grid).
+cpu_baseline: the same program as C/OpenMP on all host cores, built with gcc
+-O3 -fopenmp -ffp-contract=off, on a bounded sample (about 96 steps of the
+full 512^3 grid for the region-plan library, 20 for the plain oracle): the
+library generated on the reference's own region plan (Cut.hs restated) and
+the plain oracle are both timed; the faster is reported.
 """
 from __future__ import annotations
 
@@ -137,31 +139,47 @@ def timed_run(lib, var, lx, steps, warmup):
     return t1 - t0, ev_ms, main_ms, main_launches
 
 
-def cpu_baseline(sample_steps=20):
-    """Oracle C/OpenMP library on the host cores, 512^3 heat, `sample_steps`."""
+def _time_cpu_library(lib, cells, per, sample_steps):
     import numpy as np
-    from oracle.build_oracle import ORACLE_ONLY, compiled_extra, oracle_path
-    from formura_amd.runtime import StepLibrary
-    name = "heat512_cpu"
-    c = compiled_extra(name)
-    path = oracle_path(name)
-    lib = StepLibrary(path, c.layout, device=False)
-    lib.init()
     rng = np.random.default_rng(1)
-    lib.set_state({"u": rng.standard_normal(tuple(c.layout.intra))})
-    per = c.layout.steps_per_forward
+    lib.set_state({"u": rng.standard_normal(tuple(lib.layout.intra))})
     lib.forward()                       # warm-up (first touch, thread start)
     n = max(1, sample_steps // per)
     t0 = time.perf_counter()
     for _ in range(n):
         lib.forward()
     dt = time.perf_counter() - t0
-    cells = int(np.prod(c.layout.intra))
+    return n * per * cells / dt / 1e9, n * per, dt
+
+
+def cpu_baseline(sample_steps=96, plain_steps=20):
+    """Heat 512^3 FP64 on the host cores, about `sample_steps` steps of the
+    region-plan library and `plain_steps` of the plain oracle: the C generated on the reference's own region plan
+    (Cut.hs restated: 2 x 2 x 2 regions, T = 2, skewed in-place stores, ridges
+    and self-message facets) and the plain oracle (one loop nest per manifest
+    node over the whole grid).  The faster one is the baseline."""
+    import numpy as np
+    from oracle.build_oracle import compiled_extra, oracle_path, planned_layout, planned_path
+    from formura_amd.runtime import StepLibrary
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(n * per * cells / dt / 1e9, 4), "unit": "GLUP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n * per} steps of heat 512^3 FP64 (oracle C/OpenMP, gcc -O3 -fopenmp "
-                      f"-ffp-contract=off), {dt:.1f} s"}
+    runs = {}
+    c, lay = planned_layout("heat512_plan")
+    lib = StepLibrary(planned_path("heat512_plan"), lay, device=False)
+    lib.init()
+    cells = int(np.prod(lay.intra))
+    runs["region plan"] = _time_cpu_library(lib, cells, lay.steps_per_forward, sample_steps)
+    c2 = compiled_extra("heat512_cpu")
+    lib2 = StepLibrary(oracle_path("heat512_cpu"), c2.layout, device=False)
+    lib2.init()
+    runs["plain oracle"] = _time_cpu_library(lib2, cells, c2.layout.steps_per_forward, plain_steps)
+    best = max(runs, key=lambda k: runs[k][0])
+    rate, steps, dt = runs[best]
+    other = [f"{k} {v[0]:.3f} GLUP/s" for k, v in runs.items() if k != best]
+    what = ("C on the reference's region plan (Cut.hs restated: 2x2x2 regions, T=2)" if best == "region plan"
+            else "plain oracle C (one loop nest per manifest node)")
+    return {"value": round(rate, 4), "unit": "GLUP/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of heat 512^3 FP64, {what}, gcc -O3 -fopenmp -ffp-contract=off, "
+                      f"{dt:.1f} s; also timed: {'; '.join(other)}"}
 
 
 # registered program -> label of its line under "extra"

@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 from formura_amd import build as product_build  # noqa: E402
 from formura_amd.compiler.driver import compile_files  # noqa: E402
 from oracle.emit_cxx import write_oracle  # noqa: E402
+from oracle.emit_planned import write_planned  # noqa: E402
 
 OUT = os.path.join(HERE, "_build")
 CFLAGS = ["-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-std=gnu99", "-mcmodel=medium"]
@@ -40,6 +41,61 @@ ORACLE_ONLY = {
     "gs2d_p96x160": ("gs2d", {"intra_node_shape": [96, 160], "monitor_interval": 40}, _GS2D_PARAMS),
     "gs2d_p64x32": ("gs2d", {"intra_node_shape": [64, 32], "monitor_interval": 40}, _GS2D_PARAMS),
 }
+
+
+# region-plan libraries (the reference's loop and region structure, Cut.hs):
+# (registered program, numerical-config overrides, temporal blocking T).
+# T >= 2: the plan stores the state in place at a skewed position, which is
+# safe when the state updates read the state only through manifest arrays --
+# always the case for the T-fold graph with T >= 2.
+PLANNED = {
+    # the CPU-baseline sample: heat 512^3, 2 x 2 x 2 regions, T = 2, 10 steps per call
+    "heat512_plan": ("heat512", {"monitor_interval": 10}, 2),
+    # small programs for the bitwise checks against the plain oracle (regions
+    # wider than the stencil reach of the T-fold step)
+    "gs3d_plan": ("gs3d_t", {"monitor_interval": 4, "initial_walls": {"x": [12], "y": [40], "z": [64, 100]}}, 2),
+    "heat_plan": ("heat_t48", {"monitor_interval": 6, "initial_walls": {"x": [10, 30], "y": [24], "z": [36]}}, 3),
+    "gs2d_plan": ("gs2d", {"monitor_interval": 4, "initial_walls": {"x": [100], "y": [60, 180]}}, 2),
+    "heat_rk2_plan": ("heat_rk2_t", {"intra_node_shape": [32, 40, 36], "monitor_interval": 4,
+                                     "initial_walls": {"x": [16], "y": [20], "z": [18]}}, 2),
+    "gs3d_rk4_plan": ("gs3d_rk4_t", {"intra_node_shape": [40, 48, 56], "monitor_interval": 4,
+                                     "initial_walls": {"x": [20], "y": [24], "z": [28]}}, 2),
+}
+
+
+def planned_compiled(name: str):
+    base, over, T = PLANNED[name]
+    fmr, yaml_, _precision, _params, over0, _ = product_build.PROGRAMS[base]
+    ov = dict(over0 or {})
+    ov.update(over)
+    ov["temporal_blocking_interval"] = T
+    return compile_files(name, fmr, yaml_, "faithful", (), ov), T
+
+
+def planned_path(name: str) -> str:
+    return os.path.join(OUT, f"lib{name}.so")
+
+
+def planned_layout(name: str):
+    """Compiled program and the layout of its region-plan library (array
+    shapes and margins from the plan)."""
+    from oracle.emit_planned import _Emitter
+    c, T = planned_compiled(name)
+    nc = c.mm.numerical_config
+    e = _Emitter(c.layout, c.mm, T, nc.initial_walls, nc.option_strings or ["omp", "omp-collapse"])
+    return c, e.layout()
+
+
+def build_planned(name: str) -> str:
+    c, T = planned_compiled(name)
+    src, _ = write_planned(OUT, c.layout, c.mm, T)
+    out = planned_path(name)
+    cmd = ["gcc"] + CFLAGS + ["-I", OUT, "-I", product_build.MPI_STUB, src, "-o", out + ".tmp", "-lm"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"region-plan build failed: {' '.join(cmd)}\n{r.stdout[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 def oracle_path(name: str) -> str:
@@ -76,6 +132,8 @@ def build_oracle(name: str) -> str:
 def build_all(names=None):
     names = names or (list(product_build.PROGRAMS) + list(ORACLE_ONLY))
     out = {n: build_oracle(n) for n in names}
+    for n in PLANNED:
+        out[n] = build_planned(n)
     if "gs3d64" in out:
         # the example driver, linked against the CPU library of the same program
         out["gs3d64_main"] = product_build.build_driver("gs3d64", oracle_lib=out["gs3d64"])
