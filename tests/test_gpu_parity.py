@@ -226,3 +226,76 @@ def test_large_multistage_program_bitwise():
     lib.set_state(st)
     lib.forward()
     assert_bitwise(lib.get_state(), oracle_run("fveuler_t", st, 1), "fveuler_t Formura_Forward")
+
+
+def extra_oracle(name):
+    key = name + "/cpu"
+    if key not in _cache:
+        from oracle.build_oracle import compiled_extra, oracle_path
+        c = compiled_extra(name)
+        lib = StepLibrary(oracle_path(name), c.layout, device=False)
+        lib.init()
+        _cache[key] = lib
+    return _cache[key]
+
+
+@pytest.mark.parametrize("variant", [(KIND_MARCH, 3, 17, 1, 1, 1, 32), (KIND_MARCH, 4, 14, 4, 1, 1, -1)])
+def test_heat512_full_size_bitwise(variant):
+    """The headline workload at its full size: heat 512^3 FP64, 10 steps of
+    the march variants the benchmark's search picks (T=3 and T=4, so 3+3+3+1
+    and 4+4+1+1 launches), against the oracle's 10 steps, bit for bit."""
+    lib = hip("heat512")
+    o = extra_oracle("heat512_cpu")
+    assert o.layout.steps_per_forward == 10
+    st = heat_state(lib, 512)
+    lib.set_kernel(*variant[:6])
+    lib.set_chunk(variant[6])
+    _say(f"[parity] heat512 full size, variant {variant}, 10 steps")
+    got = device_run(lib, st, 10)
+    o.set_state(st)
+    o.forward()
+    assert_bitwise(got, o.get_state(), f"heat512 {variant}")
+    lib.set_chunk(64)
+
+
+def test_heat512_fourier_mode_and_conservation():
+    """Size-independent checks at 512^3 over 48 steps (default kernel): a
+    Fourier mode is scaled per step by the closed-form factor
+    1 + D dt / dx^2 * sum_a (2 cos(2 pi k_a / N) - 2) (relative L2 <= 1e-12),
+    and the FP64 sum of a random field is conserved (<= 1e-9 of sum |u|)."""
+    lib = hip("heat512")
+    n = tuple(lib.layout.intra)
+    k = (3, 5, 7)
+    x = [np.arange(m) for m in n]
+    mode = (np.sin(2 * np.pi * k[0] * x[0] / n[0])[:, None, None]
+            * np.sin(2 * np.pi * k[1] * x[1] / n[1])[None, :, None]
+            * np.sin(2 * np.pi * k[2] * x[2] / n[2])[None, None, :])
+    lam = 1.0 + 0.1 * sum(2 * np.cos(2 * np.pi * ka / m) - 2 for ka, m in zip(k, n))
+    _say(f"[parity] heat512 Fourier mode {k}, 48 steps, factor {lam:.15f}")
+    got = device_run(lib, {"u": mode}, 48)["u"]
+    want = lam ** 48 * mode
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    _say(f"[parity] heat512 Fourier mode rel L2 {err:.3e}")
+    assert err <= 1e-12
+    st = heat_state(lib, 77)
+    lib.set_state(st)
+    lib.upload()
+    s0, _ = lib.reduce(0)
+    lib.forward_device(48)
+    s1, _ = lib.reduce(0)
+    _say(f"[parity] heat512 sum before {s0:.12e} after {s1:.12e}")
+    assert abs(s1 - s0) <= 1e-9 * np.abs(st["u"]).sum()
+
+
+def test_grayscott512_full_size_bitwise():
+    """Gray-Scott 512^3 FP64 at full size: 4 steps of the default kernel
+    against the oracle's 4 steps, bit for bit."""
+    lib = hip("gs3d512")
+    o = extra_oracle("gs3d512_cpu")
+    assert o.layout.steps_per_forward == 4
+    st = gs_state(lib, 99)
+    _say(f"[parity] gs3d512 full size, kernel {lib.get_kernel()}, 4 steps")
+    got = device_run(lib, st, 4)
+    o.set_state(st)
+    o.forward()
+    assert_bitwise(got, o.get_state(), "gs3d512")
