@@ -309,3 +309,46 @@ def test_reference_mhd_programs_compile_to_stage_kernels(reference_tree):
         src = emit_hip(c.layout, c.ir, "faithful", [], st)
         assert "static constexpr bool INLINED = false;" in src
         assert src.count("&fmr::launch_stage<Prog, Stage") == len(groups)
+
+
+# examples of the reference tree and what the current grammar says of them
+REFERENCE_EXAMPLES_OK = [
+    "2d-mhd-dc.fmr", "2d-mhd.fmr", "3d-localsun.fmr", "3d-mhd-lessM.fmr", "3d-mhd-rk4.fmr", "av-mhd.fmr",
+    "champion.fmr", "diffusion.fmr", "finalist.fmr", "lotka-volterra.fmr", "pearson-2d.fmr",
+    "pearson-3d-0M-alt.fmr", "pearson-3d-0M.fmr", "pearson-3d-fast.fmr", "pearson-3d-lap.fmr",
+    "pearson-3d.fmr", "pearson-3dx2.fmr", "pearson.fmr", "rule30.fmr",
+]
+# written for an older grammar: numerical config inside the .fmr
+# (`intra_node_shape :: ...`) or `for(x)` lambdas; the current keyword set
+# (Parser.hs:80-83) has `fun` and no such declarations
+REFERENCE_EXAMPLES_OLD_GRAMMAR = ["av-mhd-3d.fmr", "hydro.fmr", "mhd.fmr", "diffusion-long.fmr"]
+
+
+def _compile_reference_example(reference_tree, name):
+    from formura_amd.compiler.driver import compile_source
+    ex = os.path.join(reference_tree, "examples")
+    src = open(os.path.join(ex, name), encoding="utf-8").read()
+    dim = int(re.search(r"dimension\s*::\s*(\d+)", src).group(1))
+    y = os.path.join(ex, name[:-4] + ".yaml")
+    nc = NumericalConfig.from_file(y) if os.path.exists(y) else NumericalConfig(
+        intra_node_shape=[32] * dim, mpi_grid_shape=[1] * dim, monitor_interval=2)
+    nc.intra_node_shape = [min(64, n) for n in nc.intra_node_shape]
+    nc.mpi_grid_shape = [1] * dim
+    return compile_source("ex", src, nc)
+
+
+def test_reference_examples_compile(reference_tree):
+    """Every example of the reference tree written in the current grammar goes
+    through the front end, the inlining to a point IR and the stage lowering
+    (one rank, grids capped at 64 per axis)."""
+    from formura_amd.compiler.pointir import lower_stages
+    for name in REFERENCE_EXAMPLES_OK:
+        c = _compile_reference_example(reference_tree, name)
+        assert c.ir.ops and c.layout.field_names, name
+        assert lower_stages(c.mm), name
+
+
+def test_reference_examples_in_the_old_grammar_are_rejected(reference_tree):
+    for name in REFERENCE_EXAMPLES_OLD_GRAMMAR:
+        with pytest.raises((FormuraSyntaxError, FormuraCompileError)):
+            _compile_reference_example(reference_tree, name)
