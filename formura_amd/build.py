@@ -138,18 +138,28 @@ def resources(name: str) -> Dict[tuple, Dict[str, int]]:
 
 
 def build_program(name: str) -> str:
-    """Generate and compile one program; march variants whose registers spill
-    to scratch memory are dropped and the library is compiled again without
-    them."""
+    """Generate and compile one program.  A march variant whose registers
+    spill to scratch memory is compiled again with one row per wave fewer;
+    if that spills too it is dropped."""
     c, _, src = generate(name)
     out = compile_hip(src, lib_path(name))
-    spilled = [k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0]
+    spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}
     if spilled:
         from formura_amd.compiler.driver import write_hip
         from formura_amd.compiler.emit_hip import variants_for
-        keep = [v for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision)) if tuple(v) not in spilled]
-        _, src = write_hip(c, BUILD, keep)
+        base = [tuple(v) for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision))]
+        trial = []
+        for v in base:
+            if v not in spilled:
+                trial.append(v)
+            elif v[1] - 1 > 2 * v[0] // v[2] and (v[0], v[1] - 1) + v[2:] not in base:
+                trial.append((v[0], v[1] - 1) + v[2:])
+        _, src = write_hip(c, BUILD, trial)
         out = compile_hip(src, lib_path(name))
+        spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}
+        if spilled:
+            _, src = write_hip(c, BUILD, [v for v in trial if v not in spilled])
+            out = compile_hip(src, lib_path(name))
     return out
 
 

@@ -61,9 +61,13 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
         their edge rows through LDS."""
     words = 2 if real == "double" else 1
     if max_t is None:
-        max_t = 5 if real == "double" else 8
+        # FP64: 1..5, and 8 for the sweep over {1, 2, 4, 8} (only a
+        # multi-wave tile fits the registers there)
+        depths = list(range(1, 6)) + [8] if real == "double" else list(range(1, 9))
+    else:
+        depths = list(range(1, max_t + 1))
     out = []
-    for T in range(1, max_t + 1):
+    for T in depths:
         cand = []
         for budget, pf, occ in ((AGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 1, 2), (OCC2_REG_BUDGET, 2, 2)):
             by = min(_max_rows(T, nf, words, pf, budget), 64 + 2 * T)
