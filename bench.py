@@ -90,7 +90,7 @@ def candidates(lib):
         for rows in (4, 16):     # wavefronts per workgroup of the stage kernels
             out.append(((KIND_STAGED, 1, rows, 1, 1, 1), 64))
     for v in lib.list_kernels():
-        for lx in (32, 64, 128, -1, -2):   # run length, or -workgroups per SIMD
+        for lx in (32, 64, 128, -1, -2) + ((-4,) if v[4] >= 4 else ()):   # run length, or -workgroups per SIMD
             out.append(((KIND_MARCH,) + tuple(v), lx))
     return out
 

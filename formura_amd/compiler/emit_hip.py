@@ -36,6 +36,7 @@ AGPR_REG_BUDGET = 340    # one wave per SIMD, some window values parked in AGPRs
 # operations per site gets stage kernels only (3D MHD inlined: ~550 000).
 INLINE_LIMIT = 6000
 OCC2_REG_BUDGET = 236    # two waves per SIMD (256 registers each, some slack for the scheduler)
+OCC4_REG_BUDGET = 112    # four waves per SIMD (128 registers each)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 
 
@@ -56,7 +57,7 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
       * one wave per tile (no LDS, no barrier) with as many rows as fit the
         AGPR-assisted budget, one wave per SIMD;
       * one wave per tile at two waves per SIMD, prefetching one and two
-        planes ahead;
+        planes ahead (and, for T <= 2, at four waves per SIMD);
       * a tile of up to 64 + 2T rows split over up to 4 waves that exchange
         their edge rows through LDS."""
     words = 2 if real == "double" else 1
@@ -69,7 +70,10 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
     out = []
     for T in depths:
         cand = []
-        for budget, pf, occ in ((AGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 1, 2), (OCC2_REG_BUDGET, 2, 2)):
+        shapes = [(AGPR_REG_BUDGET, 1, 1), (OCC2_REG_BUDGET, 1, 2), (OCC2_REG_BUDGET, 2, 2)]
+        if T <= 2:      # shallow fusion is latency-bound: more waves in flight
+            shapes += [(OCC4_REG_BUDGET, 1, 4), (OCC4_REG_BUDGET, 2, 4)]
+        for budget, pf, occ in shapes:
             by = min(_max_rows(T, nf, words, pf, budget), 64 + 2 * T)
             if by >= 2 * T + 2:
                 cand.append((T, by, 1, pf, occ))

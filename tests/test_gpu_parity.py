@@ -106,7 +106,7 @@ def test_every_kernel_variant_bitwise(name, mk):
     for kind, T, by, wv, pf, occ in variants:
         # runs of 64 or 7 planes (7: runs cross tile boundaries), and one or two
         # workgroups per SIMD (runs of a plane or two on these small grids)
-        for lx in ((64, 7, -1, -2) if kind == KIND_MARCH else (64,)):
+        for lx in ((64, 7, -1, -2) + ((-4,) if occ >= 4 else ()) if kind == KIND_MARCH else (64,)):
             label = f"{name} kind={kind} T={T} rows={by} waves={wv} pf={pf} occ={occ} lx={lx}"
             _say(f"[parity] {label} steps={steps}")
             lib.set_kernel(kind, T, by, wv, pf, occ)
