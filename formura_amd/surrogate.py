@@ -519,7 +519,7 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     out["surrogate"]["steps_by_window"] = [SUPER * min(n_super, j + CHECK_EVERY) for j in range(0, n_super, CHECK_EVERY)]
     log(f"checked surrogate: {out['surrogate']}")
     # plain stencils and the batched stencil from the same state
-    for name, key in (("gs2d_batch_f32", "plain_fp32"), ("gs2d_batch_f64", "plain_fp64")):
+    def time_plain(name, nsteps):
         lib = open_batch(name)
         if lib.specs != specs:
             lib.init(specs)
@@ -530,11 +530,29 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
         for i in range(n_traj):
             lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
         t0 = time.perf_counter()
-        lib.advance_plain(steps)
+        lib.advance_plain(nsteps)
         sec = time.perf_counter() - t0
-        res = np.stack([np.stack(lib.get_state(i)) for i in range(n_traj)])
+        return sec, np.stack([np.stack(lib.get_state(i)) for i in range(n_traj)])
+
+    for name, key in (("gs2d_batch_f32", "plain_fp32"), ("gs2d_batch_f64", "plain_fp64")):
+        sec, res = time_plain(name, steps)
         out[key] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth)}
         log(f"{key}: {out[key]}")
+    # the horizon over which the checked surrogate stays within the bound of
+    # the FP64 run, timed on its own against the plain FP64 stencil
+    sw = out["surrogate"]
+    ok = [(n, e) for n, e in zip(sw["steps_by_window"], sw["rel_l2_vs_fp64_by_window"]) if e <= bound]
+    if ok:
+        h, e_h = max(ok)
+        st.load(sur, start)
+        ro3 = checked_rollout(sur, st, fk, xs, h // SUPER, bound=bound)
+        sec64, _ = time_plain("gs2d_batch_f64", h)
+        sec32, _ = time_plain("gs2d_batch_f32", h)
+        out["within_bound"] = {"steps": h, "rel_l2_vs_fp64": e_h, "surrogate_seconds": round(ro3.seconds, 4),
+                               "plain_fp64_seconds": round(sec64, 4), "plain_fp32_seconds": round(sec32, 4),
+                               "speedup_vs_plain_fp64": round(sec64 / ro3.seconds, 2),
+                               "speedup_vs_plain_fp32": round(sec32 / ro3.seconds, 2)}
+        log(f"within the bound: {out['within_bound']}")
     st.lib.plan([0] * n_traj, [(8, 2, 32)])
     for i in range(n_traj):
         st.lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
