@@ -74,12 +74,25 @@ __host__ __device__ inline float bf2f(bf16 h) {
   return f;
 }
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26:
+// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2),
+// t = 1 / (1 + p |x|), |error| <= 1.5e-7 -- branch-free, a few VALU
+// instructions; the activations it feeds are rounded to BF16 (8 bits).
+__device__ inline float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+  const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+                      0.254829592f) * t;
+  return copysignf(1.0f - poly * __expf(-ax * ax), x);
+}
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float z) {
-  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+  const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
   return cdf + z * pdf;
 }
+// BF16 with round to nearest even by the hardware conversion (v_cvt_pk_bf16_f32)
+__device__ inline bf16 f2bf_dev(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
 __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -119,7 +132,7 @@ __global__ void pack_input(const float* __restrict__ state, const float* __restr
 //   EPI_HIDDEN: z = acc + bias; zq = bf16(z) -> zout; a = bf16(gelu(z)) -> aout
 //   EPI_OUT:    state_out = state_in + scale * (acc + bias) for n < 2
 //   EPI_DGELU:  aout = bf16(acc * gelu'(zq[p][n]))   (dz of the previous layer)
-enum { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_DGELU = 2 };
+enum { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_DGELU = 2, EPI_INFER = 3 };   // EPI_INFER: a_l only (no z_l kept)
 
 template <int CI, int NT, int DIR, int EPI>
 __global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ wt,
@@ -179,18 +192,153 @@ __global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, co
         const float s = acc[t][nt][i];
         if constexpr (EPI == EPI_HIDDEN) {
           const float z = s + bias[n];
-          zout[p * CO + n] = f2bf(z);
-          aout[p * CO + n] = f2bf(gelu(z));
+          zout[p * CO + n] = f2bf_dev(z);
+          aout[p * CO + n] = f2bf_dev(gelu(z));
+        } else if constexpr (EPI == EPI_INFER) {
+          aout[p * CO + n] = f2bf_dev(gelu(s + bias[n]));
         } else if constexpr (EPI == EPI_OUT) {
           if (n < 2) {
             const size_t q = ((size_t)b * 2 + n) * H * W + (size_t)y * W + x;
             state_out[q] = state_in[q] + scale[n] * (s + bias[n]);
           }
         } else {
-          aout[p * CO + n] = f2bf(s * gelu_grad(bf2f(zq[p * CO + n])));
+          aout[p * CO + n] = f2bf_dev(s * gelu_grad(bf2f(zq[p * CO + n])));
         }
       }
     }
+}
+
+// The same convolution with the input tile staged in LDS: a workgroup of 4
+// waves covers 4*RPW image rows x 64 pixels; it first copies rows
+// [y0-1, y0+4*RPW] x pixels [x0-1, x0+64] (periodic wrap) of all CI channels
+// into LDS, then each wave computes RPW rows (4*RPW M tiles), reading its A
+// fragments from LDS and each B fragment (weights, from global memory) once
+// for all its M tiles.  Pixel stride in LDS: CI + PAD channels (PAD = 8: 16
+// bytes of padding spread the 16 lanes of a fragment over the banks).
+template <int CI, int NT, int DIR, int EPI, int RPW, int PAD>
+__global__ void __launch_bounds__(256) conv_lds(const bf16* __restrict__ in, const bf16* __restrict__ wt,
+                                                const float* __restrict__ bias, bf16* __restrict__ zout,
+                                                bf16* __restrict__ aout, const bf16* __restrict__ zq,
+                                                const float* __restrict__ state_in, float* __restrict__ state_out,
+                                                const float* __restrict__ scale, int B, int H, int W, int CO) {
+  constexpr int KSTEPS = (9 * CI + 31) / 32;
+  constexpr int TR = 4 * RPW + 2;          // tile rows in LDS
+  constexpr int TP = 66;                   // tile pixels in LDS
+  constexpr int CS = CI + PAD;             // channel stride of one pixel in LDS
+  __shared__ __attribute__((aligned(16))) bf16 tile[TR * TP * CS];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int segs = W / 64, bands = H / (4 * RPW);
+  const int blk = blockIdx.x;              // (b, band, segment)
+  const int b = blk / (bands * segs);
+  const int y0 = ((blk / segs) % bands) * 4 * RPW;
+  const int x0 = (blk % segs) * 64;
+  const size_t img = (size_t)b * H * W;
+  // stage: TR x TP pixels x CI channels, 8 channels (16 bytes) per copy
+  constexpr int CH8 = CI / 8;
+  for (int i = threadIdx.x; i < TR * TP * CH8; i += 256) {
+    const int c8 = i % CH8, px = (i / CH8) % TP, r = i / (CH8 * TP);
+    const int yy = wrapi(y0 - 1 + r, H), xx = wrapi(x0 - 1 + px, W);
+    *(bf16x8*)(tile + (r * TP + px) * CS + c8 * 8) = *(const bf16x8*)(in + (img + (size_t)yy * W + xx) * CI + c8 * 8);
+  }
+  __syncthreads();
+  const int m = lane % 16, g = lane / 16;
+  f32x4 acc[RPW][4][NT];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[r][t][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // weight row n is [9][CI] = the K index k itself; the A fragment of lane
+  // (m, g) at k-step ks sits at a lane base plus a per-k offset, and the
+  // (row, M tile) offsets are constants
+  const bf16* wrow[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) wrow[nt] = wt + (size_t)(nt * 16 + m) * 9 * CI + g * 8;
+  const bf16* abase = tile + ((wave * RPW + 1) * TP + m + 1) * CS;
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int k = ks * 32 + g * 8;
+    const bool kvalid = k < 9 * CI;
+    const int tap = kvalid ? k / CI : 0;
+    const int c = k - tap * CI;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const bf16* ap = abase + (DIR * dy * TP + DIR * dx) * CS + c;
+    bf16x8 bfr[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      bfr[nt] = kvalid ? *(const bf16x8*)(wrow[nt] + ks * 32) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 afr = kvalid ? *(const bf16x8*)(ap + (r * TP + t * 16) * CS) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[r][t][nt] = mfma_bf16(afr, bfr[nt], acc[r][t][nt]);
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int y = y0 + wave * RPW + r;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int n = nt * 16 + m;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int x = x0 + t * 16 + 4 * g + i;
+          const size_t p = img + (size_t)y * W + x;
+          const float s = acc[r][t][nt][i];
+          if constexpr (EPI == EPI_HIDDEN) {
+            const float z = s + bias[n];
+            zout[p * CO + n] = f2bf_dev(z);
+            aout[p * CO + n] = f2bf_dev(gelu(z));
+          } else if constexpr (EPI == EPI_INFER) {
+            aout[p * CO + n] = f2bf_dev(gelu(s + bias[n]));
+          } else if constexpr (EPI == EPI_OUT) {
+            if (n < 2) {
+              const size_t q = ((size_t)b * 2 + n) * H * W + (size_t)y * W + x;
+              state_out[q] = state_in[q] + scale[n] * (s + bias[n]);
+            }
+          } else {
+            aout[p * CO + n] = f2bf_dev(s * gelu_grad(bf2f(zq[p * CO + n])));
+          }
+        }
+      }
+  }
+}
+
+// per instance: FP64 sums of (a - b)^2 and b^2 over its n_each floats, in a
+// fixed order (block = (chunk, instance), 256 threads, then sum_pairs)
+constexpr int DIFF_CHUNKS = 16;
+__global__ void __launch_bounds__(256) sq_diff(const float* __restrict__ a, const float* __restrict__ b,
+                                               size_t n_each, double* __restrict__ part) {
+  __shared__ double sd[256], sb[256];
+  const int inst = blockIdx.y, chunk = blockIdx.x;
+  const size_t lo = n_each * chunk / DIFF_CHUNKS, hi = n_each * (chunk + 1) / DIFF_CHUNKS;
+  const float* pa = a + (size_t)inst * n_each;
+  const float* pb = b + (size_t)inst * n_each;
+  double d = 0.0, r = 0.0;
+  for (size_t i = lo + threadIdx.x; i < hi; i += 256) {
+    const double x = pa[i], y = pb[i];
+    d += (x - y) * (x - y);
+    r += y * y;
+  }
+  sd[threadIdx.x] = d;
+  sb[threadIdx.x] = r;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[((size_t)inst * DIFF_CHUNKS + chunk) * 2] = sd[0];
+    part[((size_t)inst * DIFF_CHUNKS + chunk) * 2 + 1] = sb[0];
+  }
 }
 
 // loss and the output gradient: dz5[p][n] = 2 s_n (out - target) / count for
@@ -376,6 +524,7 @@ struct Net {
   bf16 *x0 = nullptr, *z[4], *a[4], *dz5 = nullptr, *dza = nullptr, *dzb = nullptr;
   float *part = nullptr, *out = nullptr;
   double* lpart = nullptr;
+  double* dpart = nullptr;   // sq_diff partial sums: maxB x DIFF_CHUNKS x 2
   size_t part_floats = 0;
   hipStream_t stream = nullptr;
   int adam_t = 0;
@@ -399,20 +548,46 @@ void cast_all() {
   SCHECK(hipGetLastError());
 }
 
-void forward(const float* state, const float* fk, float* out_state, int B) {
+// convolution kernel: 0 = fragments from global memory (conv_mfma), 1 = LDS
+// tile of 4 rows (default), 2 = LDS tile of 8 rows (conv_lds; padded pixels)
+static int conv_kind = 1;
+
+template <int CI, int NT, int DIR, int EPI>
+void launch_conv(const bf16* in, const bf16* wt, const float* bias, bf16* z, bf16* a, const bf16* zq,
+                 const float* s_in, float* s_out, const float* scale, int B, int H, int W, int CO) {
+  if (conv_kind == 2 && H % 8 == 0) {
+    hipLaunchKernelGGL((conv_lds<CI, NT, DIR, EPI, 2, 8>), dim3(B * (H / 8) * (W / 64)), dim3(256), 0, net.stream,
+                       in, wt, bias, z, a, zq, s_in, s_out, scale, B, H, W, CO);
+  } else if (conv_kind == 1 && H % 4 == 0) {
+    hipLaunchKernelGGL((conv_lds<CI, NT, DIR, EPI, 1, 8>), dim3(B * (H / 4) * (W / 64)), dim3(256), 0, net.stream,
+                       in, wt, bias, z, a, zq, s_in, s_out, scale, B, H, W, CO);
+  } else {
+    hipLaunchKernelGGL((conv_mfma<CI, NT, DIR, EPI>), dim3((B * H * (W / 64) + 3) / 4), dim3(256), 0, net.stream,
+                       in, wt, bias, z, a, zq, s_in, s_out, scale, B, H, W, CO);
+  }
+}
+
+// train: keep the pre-activations z_l for the backward pass
+void forward(const float* state, const float* fk, float* out_state, int B, bool train = true) {
   const int H = net.H, W = net.W;
   const size_t px = (size_t)B * H * W;
   hipLaunchKernelGGL(pack_input, dim3(grid_for(px, 256)), dim3(256), 0, net.stream, state, fk, net.x0, B, H, W);
-  const int blocks = (int)((B * H * (W / 64) + 3) / 4);
   const bf16* in = net.x0;
-  hipLaunchKernelGGL((conv_mfma<8, 3, 1, EPI_HIDDEN>), dim3(blocks), dim3(256), 0, net.stream, in, net.wf[0],
-                     net.w + net.off_b[0], net.z[0], net.a[0], nullptr, nullptr, nullptr, nullptr, B, H, W, 48);
-  for (int l = 1; l < 4; ++l)
-    hipLaunchKernelGGL((conv_mfma<48, 3, 1, EPI_HIDDEN>), dim3(blocks), dim3(256), 0, net.stream, net.a[l - 1],
-                       net.wf[l], net.w + net.off_b[l], net.z[l], net.a[l], nullptr, nullptr, nullptr, nullptr, B, H,
-                       W, 48);
-  hipLaunchKernelGGL((conv_mfma<48, 1, 1, EPI_OUT>), dim3(blocks), dim3(256), 0, net.stream, net.a[3], net.wf[4],
-                     net.w + net.off_b[4], nullptr, nullptr, nullptr, state, out_state, net.scale, B, H, W, 8);
+  if (train) {
+    launch_conv<8, 3, 1, EPI_HIDDEN>(in, net.wf[0], net.w + net.off_b[0], net.z[0], net.a[0], nullptr, nullptr,
+                                     nullptr, nullptr, B, H, W, 48);
+    for (int l = 1; l < 4; ++l)
+      launch_conv<48, 3, 1, EPI_HIDDEN>(net.a[l - 1], net.wf[l], net.w + net.off_b[l], net.z[l], net.a[l], nullptr,
+                                        nullptr, nullptr, nullptr, B, H, W, 48);
+  } else {
+    launch_conv<8, 3, 1, EPI_INFER>(in, net.wf[0], net.w + net.off_b[0], nullptr, net.a[0], nullptr, nullptr,
+                                    nullptr, nullptr, B, H, W, 48);
+    for (int l = 1; l < 4; ++l)
+      launch_conv<48, 3, 1, EPI_INFER>(net.a[l - 1], net.wf[l], net.w + net.off_b[l], nullptr, net.a[l], nullptr,
+                                       nullptr, nullptr, nullptr, B, H, W, 48);
+  }
+  launch_conv<48, 1, 1, EPI_OUT>(net.a[3], net.wf[4], net.w + net.off_b[4], nullptr, nullptr, nullptr, state,
+                                 out_state, net.scale, B, H, W, 8);
   SCHECK(hipGetLastError());
 }
 
@@ -467,11 +642,11 @@ double backward(const float* target, int B) {
     // data gradient into the previous layer (times gelu')
     if (l > 0) {
       if (l == NL - 1)
-        hipLaunchKernelGGL((conv_mfma<8, 3, -1, EPI_DGELU>), dim3(blocks), dim3(256), 0, net.stream, dz, net.wb[l],
-                           nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr, nullptr, B, H, W, 48);
+        launch_conv<8, 3, -1, EPI_DGELU>(dz, net.wb[l], nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr, nullptr,
+                                         B, H, W, 48);
       else
-        hipLaunchKernelGGL((conv_mfma<48, 3, -1, EPI_DGELU>), dim3(blocks), dim3(256), 0, net.stream, dz,
-                           net.wb[l], nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr, nullptr, B, H, W, 48);
+        launch_conv<48, 3, -1, EPI_DGELU>(dz, net.wb[l], nullptr, nullptr, dzn, net.z[l - 1], nullptr, nullptr,
+                                          nullptr, B, H, W, 48);
       dz = dzn;
       dzn = (dzn == net.dza) ? net.dzb : net.dza;
     }
@@ -536,6 +711,7 @@ int Sur_Init(int H, int W, int maxB) {
   net.part_floats = (size_t)chunks * 48 * 9 * 48 + (size_t)48 * 9 * 48;
   SCHECK(hipMalloc(&net.part, sizeof(float) * net.part_floats));
   SCHECK(hipMalloc(&net.lpart, sizeof(double) * 1024));
+  SCHECK(hipMalloc(&net.dpart, sizeof(double) * maxB * DIFF_CHUNKS * 2));
   const float one[2] = {1.f, 1.f};
   SCHECK(hipMemcpyAsync(net.scale, one, sizeof(one), hipMemcpyHostToDevice, net.stream));
   SCHECK(hipStreamSynchronize(net.stream));
@@ -575,10 +751,39 @@ int Sur_Set_Scale(const float* s2) {
   return 0;
 }
 
+// convolution kernel choice (0: fragments from global memory, 1: LDS tile of
+// 4 rows (default), 2: LDS tile of 8 rows; all compute the same sums in the
+// same order)
+int Sur_Set_Conv(int kind) {
+  if (kind < 0 || kind > 2) return -1;
+  conv_kind = kind;
+  return 0;
+}
+
+// relative L2 distance of a from b per instance (B instances of n_each
+// floats each, device pointers): rel[i] = ||a_i - b_i|| / ||b_i|| (FP64 sums)
+int Sur_Rel_L2(const float* a, const float* b, int B, size_t n_each, double* rel) {
+  if (!net.ready || B < 1 || B > net.maxB) return -1;
+  hipLaunchKernelGGL(sq_diff, dim3(DIFF_CHUNKS, B), dim3(256), 0, net.stream, a, b, n_each, net.dpart);
+  SCHECK(hipGetLastError());
+  std::vector<double> part((size_t)B * DIFF_CHUNKS * 2);
+  SCHECK(hipMemcpyAsync(part.data(), net.dpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  for (int i = 0; i < B; ++i) {
+    double d = 0.0, r = 0.0;
+    for (int c = 0; c < DIFF_CHUNKS; ++c) {
+      d += part[((size_t)i * DIFF_CHUNKS + c) * 2];
+      r += part[((size_t)i * DIFF_CHUNKS + c) * 2 + 1];
+    }
+    rel[i] = sqrt(d) / (r > 0.0 ? sqrt(r) : 1e-300);
+  }
+  return 0;
+}
+
 // state, fk, out: device pointers. state/out [B][2][H][W] FP32, fk [B][2] FP32
 int Sur_Forward(const float* state, const float* fk, float* out, int B) {
   if (!net.ready || B < 1 || B > net.maxB) return -1;
-  forward(state, fk, out, B);
+  forward(state, fk, out, B, false);
   SCHECK(hipStreamSynchronize(net.stream));
   return 0;
 }
@@ -647,7 +852,7 @@ float Sur_Time_Forward(const float* state, const float* fk, float* out, int B, i
   SCHECK(hipEventCreate(&e0));
   SCHECK(hipEventCreate(&e1));
   SCHECK(hipEventRecord(e0, net.stream));
-  for (int r = 0; r < reps; ++r) forward(state, fk, out, B);
+  for (int r = 0; r < reps; ++r) forward(state, fk, out, B, false);
   SCHECK(hipEventRecord(e1, net.stream));
   SCHECK(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -97,6 +97,8 @@ class Surrogate:
         L.Sur_Time_Forward.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.Sur_Time_Forward.restype = ctypes.c_float
         L.Sur_Add_Noise.argtypes = [vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_ulonglong]
+        L.Sur_Set_Conv.argtypes = [ctypes.c_int]
+        L.Sur_Rel_L2.argtypes = [vp, vp, ctypes.c_int, ctypes.c_size_t, dp]
         self.n = L.Sur_Num_Params()
         assert self.n == num_params()
         self.max_batch = max_batch
@@ -148,6 +150,19 @@ class Surrogate:
         if self.lib.Sur_Train_Step(state.ptr, fk.ptr, target.ptr, B, ctypes.c_float(lr), ctypes.byref(loss)) != 0:
             raise ValueError("bad train call")
         return loss.value
+
+    def rel_l2(self, a: DeviceBuffer, b: DeviceBuffer, B: int, n_each: int) -> List[float]:
+        """Per instance ||a_i - b_i|| / ||b_i|| on the device (FP64 sums)."""
+        out = (ctypes.c_double * B)()
+        if self.lib.Sur_Rel_L2(a.ptr, b.ptr, B, ctypes.c_size_t(n_each), out) != 0:
+            raise ValueError("bad rel_l2 call")
+        return [float(x) for x in out]
+
+    def set_conv(self, kind: int) -> None:
+        """Convolution kernel: 0 fragments from global memory, 1 LDS tile of
+        4 rows (default), 2 LDS tile of 8 rows; the same bits."""
+        if self.lib.Sur_Set_Conv(int(kind)) != 0:
+            raise ValueError("bad convolution kernel")
 
     def time_forward(self, state, fk, out, B: int, reps: int) -> float:
         return float(self.lib.Sur_Time_Forward(state.ptr, fk.ptr, out.ptr, B, reps))
@@ -432,9 +447,7 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
         st.advance(SUPER)
         st.store(sur, ref, 0, B)
         res.stencil_supersteps += 1
-        p = b.download(x0.shape)
-        q = ref.download(x0.shape)
-        err = max(rel_l2(p[i], q[i]) for i in range(B))
+        err = max(sur.rel_l2(b, ref, B, x0[0].size))
         res.checks.append(err)
         if err <= bound:
             a, ref = ref, a

@@ -16,10 +16,10 @@ from formura_amd.compiler.abi import header_text
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared(header):
+def declared(header, prefix="Formura_"):
     text = open(os.path.join(ROOT, "include", header), encoding="utf-8").read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(Formura_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text)))
 
 
 def exported(path):
@@ -52,6 +52,16 @@ def test_batch_libraries_export_the_abi():
     for name, path in zip(BATCH_PROGRAMS, built(BATCH_PROGRAMS)):
         syms = exported(path)
         assert want <= syms, f"{name} lacks {sorted(want - syms)}"
+
+
+def test_surrogate_library_exports_the_abi():
+    want = set(declared("formura_amd_surrogate.h", "Sur_"))
+    assert {"Sur_Forward", "Sur_Train_Step", "Sur_Set_Conv"} <= want
+    path = build.lib_path("surrogate")
+    if not os.path.exists(path):
+        pytest.skip("surrogate library not built")
+    syms = exported(path)
+    assert want <= syms, f"libsurrogate lacks {sorted(want - syms)}"
 
 
 def test_libraries_load_without_a_gpu():

@@ -124,3 +124,23 @@ def test_trained_network_passes_the_checks_the_untrained_one_fails():
     _say(f"[surrogate] untrained: checks {ro0.checks}, fallback {ro0.fallback_at}")
     assert ro0.fallback_at == 0 and ro0.checks[0] > S.BOUND
     fk.free()
+
+
+def test_device_relative_l2_matches_numpy():
+    """The rollout check's per-trajectory relative L2 (FP64 sums on the GPU)
+    against numpy in float64."""
+    from formura_amd import surrogate as S
+    sur = S.Surrogate.get(16)
+    rng = np.random.default_rng(4)
+    B, n = 5, 2 * 256 * 256
+    a = rng.standard_normal((B, n)).astype(np.float32)
+    b = (a + 1e-3 * rng.standard_normal((B, n))).astype(np.float32)
+    ba, bb = sur.buffer(a.nbytes), sur.buffer(b.nbytes)
+    ba.upload(a)
+    bb.upload(b)
+    got = sur.rel_l2(ba, bb, B, n)
+    want = [np.linalg.norm(a[i].astype(np.float64) - b[i]) / np.linalg.norm(b[i].astype(np.float64)) for i in range(B)]
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 1e-12 * w
+    ba.free()
+    bb.free()
