@@ -238,6 +238,14 @@ class TrainResult:
     scale: tuple = (1.0, 1.0)
 
 
+def forward_flops(H: int = 256, W: int = 256) -> float:
+    """Multiply-add FLOPs of one network pass over one H x W instance, as
+    executed: 2 * H * W * sum over layers of 9 * Cin * Cout, with the first
+    layer's input as its 8 hi/lo channels and the last layer's 2 outputs."""
+    cin = (2 * CIN[0],) + tuple(CIN[1:])
+    return 2.0 * H * W * sum(9 * ci * co for ci, co in zip(cin, COUT))
+
+
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
@@ -591,7 +599,9 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(prog="python -m formura_amd.surrogate")
-    ap.add_argument("command", choices=["train", "evaluate"])
+    ap.add_argument("command", choices=["train", "evaluate", "time-forward"])
+    ap.add_argument("--conv", type=int, default=1, help="time-forward: convolution kernel (0, 1, 2)")
+    ap.add_argument("--reps", type=int, default=20, help="time-forward: timed forward passes")
     ap.add_argument("--seconds", type=float, default=240.0)
     ap.add_argument("--trajectories", type=int, default=2000)
     ap.add_argument("--traj-steps", type=int, default=4096)
@@ -614,6 +624,22 @@ def main(argv=None):
         print(json.dumps(meta), flush=True)
         ev = evaluate(theta, res.scale, log=lambda m: print(m, flush=True))
         print(json.dumps(ev), flush=True)
+    elif a.command == "time-forward":
+        # forward passes of 16 held-out 256^2 instances with the committed weights
+        theta, scale, _ = load_weights(a.weights)
+        sur = Surrogate.get(16)
+        sur.set_params(theta)
+        sur.set_scale(*scale)
+        sur.set_conv(a.conv)
+        specs = trajectory_specs(16, 7)
+        x0 = initial_states(specs, 7000)
+        st, fk, out = sur.buffer(x0.nbytes), sur.buffer(8 * 16), sur.buffer(x0.nbytes)
+        st.upload(x0)
+        fk.upload(fk_array(specs))
+        sur.forward(st, fk, out, 16)
+        ms = sur.time_forward(st, fk, out, 16, a.reps) / a.reps
+        print(json.dumps({"conv": a.conv, "ms_per_forward": round(ms, 4), "instances": 16, "image": [256, 256],
+                          "tflops": round(16 * forward_flops() / (ms * 1e-3) / 1e12, 1)}), flush=True)
     else:
         theta, scale, meta = load_weights(a.weights)
         print(meta, flush=True)
