@@ -16,12 +16,13 @@
 // [B][2][H][W]; convolution weights BF16 [Cout][3][3][Cin] for the forward
 // pass and [Cin][3][3][Cout] for the data gradient; master weights FP32.
 //
-// Convolution as implicit GEMM on MFMA 16x16x32: M = pixels, N = output
-// channels, K = 9 * input channels ordered (dy, dx, c).  MFMA lane layout
-// (wave64): lane l holds A[m = l%16][k = 8*(l/16) + j] and B[k = 8*(l/16) + j]
-// [n = l%16] for j = 0..7, and C[m = 4*(l/16) + i][n = l%16] for i = 0..3.
-// Eight consecutive k are eight consecutive channels of one tap, so an A or B
-// fragment is one 16-byte load.
+// Convolution as implicit GEMM on MFMA 16x16x32: M = output channels (A =
+// weights), N = pixels (B = activations), K = 9 * input channels ordered
+// (dy, dx, c).  MFMA lane layout (wave64): lane l holds A[m = l%16][k =
+// 8*(l/16) + j] and B[k = 8*(l/16) + j][n = l%16] for j = 0..7, and C[m =
+// 4*(l/16) + i][n = l%16] for i = 0..3.  Eight consecutive k are eight
+// consecutive channels of one tap, so an A or B fragment is one 16-byte load,
+// and a lane's four results are four consecutive channels of one pixel.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,21 +79,34 @@ __host__ __device__ inline float bf2f(bf16 h) {
 // erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2),
 // t = 1 / (1 + p |x|), |error| <= 1.5e-7 -- branch-free, a few VALU
 // instructions; the activations it feeds are rounded to BF16 (8 bits).
+// The polynomial is evaluated by Horner's rule with fused multiply-adds.
 __device__ inline float erf_as(float x) {
   const float ax = fabsf(x);
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
-  const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
-                      0.254829592f) * t;
-  return copysignf(1.0f - poly * __expf(-ax * ax), x);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+  const float poly =
+      fmaf(fmaf(fmaf(fmaf(1.061405429f, t, -1.453152027f), t, 1.421413741f), t, -0.284496736f), t, 0.254829592f) * t;
+  return copysignf(fmaf(-poly, __expf(-ax * ax), 1.0f), x);
 }
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
+__device__ inline float gelu(float z) {
+  const float hz = 0.5f * z;
+  return fmaf(hz, erf_as(z * 0.70710678118654752f), hz);
+}
 __device__ inline float gelu_grad(float z) {
-  const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f));
+  const float cdf = fmaf(0.5f, erf_as(z * 0.70710678118654752f), 0.5f);
   const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
-  return cdf + z * pdf;
+  return fmaf(z, pdf, cdf);
 }
-// BF16 with round to nearest even by the hardware conversion (v_cvt_pk_bf16_f32)
-__device__ inline bf16 f2bf_dev(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_cvt __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// two floats -> two BF16 in one 32-bit word, low half first, rounded to
+// nearest even by the hardware conversion (one v_cvt_pk_bf16_f32)
+__device__ inline unsigned int pack_bf16x2(float lo, float hi) {
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(f32x2{lo, hi}, bf16x2_cvt));
+}
+__device__ inline u32x2 pack_bf16x4(const f32x4& v) { return u32x2{pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3])}; }
+__device__ inline float bf16_lo(unsigned int w) { return __builtin_bit_cast(float, w << 16); }
+__device__ inline float bf16_hi(unsigned int w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
 
 __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -129,10 +143,54 @@ __global__ void pack_input(const float* __restrict__ state, const float* __restr
 //   DIR = -1: data gradient, out[p][n] = sum_{d,c} wt[n][d][c] * in[p - d + 1][c]
 // `in` has CI channels (multiple of 8), `wt` is [NT*16 padded n][9][CI].
 // Epilogues:
-//   EPI_HIDDEN: z = acc + bias; zq = bf16(z) -> zout; a = bf16(gelu(z)) -> aout
-//   EPI_OUT:    state_out = state_in + scale * (acc + bias) for n < 2
+//   EPI_HIDDEN: z = acc (accumulated onto the bias); zq = bf16(z) -> zout;
+//               a = bf16(gelu(z)) -> aout
+//   EPI_OUT:    state_out = state_in + scale * acc (bias included) for n < 2
 //   EPI_DGELU:  aout = bf16(acc * gelu'(zq[p][n]))   (dz of the previous layer)
 enum { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_DGELU = 2, EPI_INFER = 3 };   // EPI_INFER: a_l only (no z_l kept)
+
+// Accumulator layout: the weights are the MFMA's A operand (rows = output
+// channels) and the pixels its B operand (columns), so lane (m, g) of a
+// 16x16 tile holds channels 4g..4g+3 of pixel m: four consecutive channels,
+// written as one 8-byte store.
+//
+// initial accumulator of channels c0..c0+3: their bias in the forward
+// epilogues (the output layer has 2 channels), zero for the data gradient
+template <int EPI>
+__device__ inline f32x4 acc_init(const float* __restrict__ bias, int c0) {
+  if constexpr (EPI == EPI_DGELU) {
+    return f32x4{0.f, 0.f, 0.f, 0.f};
+  } else if constexpr (EPI == EPI_OUT) {
+    return c0 == 0 ? f32x4{bias[0], bias[1], 0.f, 0.f} : f32x4{0.f, 0.f, 0.f, 0.f};
+  } else {
+    return f32x4{bias[c0], bias[c0 + 1], bias[c0 + 2], bias[c0 + 3]};
+  }
+}
+
+// epilogue of channels c0..c0+3 of pixel p = (b, y, x); s = the accumulator
+// (bias included)
+template <int EPI>
+__device__ inline void epilogue4(const f32x4& s, int b, int y, int x, size_t p, int c0, int H, int W, int CO,
+                                 bf16* __restrict__ zout, bf16* __restrict__ aout, const bf16* __restrict__ zq,
+                                 const float* __restrict__ state_in, float* __restrict__ state_out,
+                                 const float* __restrict__ scale) {
+  if constexpr (EPI == EPI_HIDDEN || EPI == EPI_INFER) {
+    if constexpr (EPI == EPI_HIDDEN) *(u32x2*)(zout + p * CO + c0) = pack_bf16x4(s);
+    *(u32x2*)(aout + p * CO + c0) = pack_bf16x4(f32x4{gelu(s[0]), gelu(s[1]), gelu(s[2]), gelu(s[3])});
+  } else if constexpr (EPI == EPI_OUT) {
+    if (c0 == 0) {
+      const size_t hw = (size_t)H * W;
+      const size_t q = (size_t)b * 2 * hw + (size_t)y * W + x;
+      state_out[q] = state_in[q] + scale[0] * s[0];
+      state_out[q + hw] = state_in[q + hw] + scale[1] * s[1];
+    }
+  } else {
+    const u32x2 zw = *(const u32x2*)(zq + p * CO + c0);
+    *(u32x2*)(aout + p * CO + c0) =
+        pack_bf16x4(f32x4{s[0] * gelu_grad(bf16_lo(zw[0])), s[1] * gelu_grad(bf16_hi(zw[0])),
+                          s[2] * gelu_grad(bf16_lo(zw[1])), s[3] * gelu_grad(bf16_hi(zw[1]))});
+  }
+}
 
 template <int CI, int NT, int DIR, int EPI>
 __global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ wt,
@@ -151,9 +209,11 @@ __global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, co
   const int m = lane % 16, g = lane / 16;
   f32x4 acc[4][NT];
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int nt = 0; nt < NT; ++nt) {
+    const f32x4 a0 = acc_init<EPI>(bias, nt * 16 + 4 * g);
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[t][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 4; ++t) acc[t][nt] = a0;
+  }
   const size_t img = (size_t)b * H * W;
 #pragma unroll 2
   for (int ks = 0; ks < KSTEPS; ++ks) {
@@ -176,36 +236,18 @@ __global__ void __launch_bounds__(256) conv_mfma(const bf16* __restrict__ in, co
                                 : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        acc[t][nt] = mfma_bf16(afr, bfr[nt], acc[t][nt]);
+        acc[t][nt] = mfma_bf16(bfr[nt], afr, acc[t][nt]);
     }
   }
-  // epilogue: lane holds pixels x0 + t*16 + 4*g + i, channel nt*16 + m
+  // epilogue: lane holds channels nt*16 + 4*g + i of pixel x0 + t*16 + m
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < 4; ++t) {
+    const int x = x0 + t * 16 + m;
+    const size_t p = img + (size_t)y * W + x;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int n = nt * 16 + m;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int x = x0 + t * 16 + 4 * g + i;
-        const size_t p = img + (size_t)y * W + x;
-        const float s = acc[t][nt][i];
-        if constexpr (EPI == EPI_HIDDEN) {
-          const float z = s + bias[n];
-          zout[p * CO + n] = f2bf_dev(z);
-          aout[p * CO + n] = f2bf_dev(gelu(z));
-        } else if constexpr (EPI == EPI_INFER) {
-          aout[p * CO + n] = f2bf_dev(gelu(s + bias[n]));
-        } else if constexpr (EPI == EPI_OUT) {
-          if (n < 2) {
-            const size_t q = ((size_t)b * 2 + n) * H * W + (size_t)y * W + x;
-            state_out[q] = state_in[q] + scale[n] * (s + bias[n]);
-          }
-        } else {
-          aout[p * CO + n] = f2bf_dev(s * gelu_grad(bf2f(zq[p * CO + n])));
-        }
-      }
-    }
+    for (int nt = 0; nt < NT; ++nt)
+      epilogue4<EPI>(acc[t][nt], b, y, x, p, nt * 16 + 4 * g, H, W, CO, zout, aout, zq, state_in, state_out, scale);
+  }
 }
 
 // The same convolution with the input tile staged in LDS: a workgroup of 4
@@ -233,22 +275,39 @@ __global__ void __launch_bounds__(256) conv_lds(const bf16* __restrict__ in, con
   const int y0 = ((blk / segs) % bands) * 4 * RPW;
   const int x0 = (blk % segs) * 64;
   const size_t img = (size_t)b * H * W;
-  // stage: TR x TP pixels x CI channels, 8 channels (16 bytes) per copy
+  // stage: TR x TP pixels x CI channels, 8 channels (16 bytes) per copy; all
+  // of a thread's loads are issued before its first LDS write (32-bit element
+  // offsets within the image: H * W * CI < 2^31, checked at init)
   constexpr int CH8 = CI / 8;
-  for (int i = threadIdx.x; i < TR * TP * CH8; i += 256) {
-    const int c8 = i % CH8, px = (i / CH8) % TP, r = i / (CH8 * TP);
-    const int yy = wrapi(y0 - 1 + r, H), xx = wrapi(x0 - 1 + px, W);
-    *(bf16x8*)(tile + (r * TP + px) * CS + c8 * 8) = *(const bf16x8*)(in + (img + (size_t)yy * W + xx) * CI + c8 * 8);
+  constexpr int NCH = TR * TP * CH8, NIT = (NCH + 255) / 256;
+  const bf16* inb = in + img * CI;
+  bf16x8 stg[NIT];
+  int loff[NIT];
+#pragma unroll
+  for (int j = 0; j < NIT; ++j) {
+    const int i = threadIdx.x + 256 * j;
+    if (NCH % 256 == 0 || j < NIT - 1 || i < NCH) {
+      const int r = i / (TP * CH8), q = i - r * (TP * CH8);
+      const int px = q / CH8, c8 = q - px * CH8;
+      const int yy = wrapi(y0 - 1 + r, H), xx = wrapi(x0 - 1 + px, W);
+      stg[j] = *(const bf16x8*)(inb + (unsigned)((yy * W + xx) * CI + c8 * 8));
+      loff[j] = (r * TP + px) * CS + c8 * 8;
+    }
   }
+#pragma unroll
+  for (int j = 0; j < NIT; ++j)
+    if (NCH % 256 == 0 || j < NIT - 1 || threadIdx.x + 256 * j < NCH) *(bf16x8*)(tile + loff[j]) = stg[j];
   __syncthreads();
   const int m = lane % 16, g = lane / 16;
   f32x4 acc[RPW][4][NT];
 #pragma unroll
-  for (int r = 0; r < RPW; ++r)
+  for (int nt = 0; nt < NT; ++nt) {
+    const f32x4 a0 = acc_init<EPI>(bias, nt * 16 + 4 * g);
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int r = 0; r < RPW; ++r)
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[r][t][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < 4; ++t) acc[r][t][nt] = a0;
+  }
   // weight row n is [9][CI] = the K index k itself; the A fragment of lane
   // (m, g) at k-step ks sits at a lane base plus a per-k offset, and the
   // (row, M tile) offsets are constants
@@ -274,38 +333,22 @@ __global__ void __launch_bounds__(256) conv_lds(const bf16* __restrict__ in, con
       for (int t = 0; t < 4; ++t) {
         const bf16x8 afr = kvalid ? *(const bf16x8*)(ap + (r * TP + t * 16) * CS) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[r][t][nt] = mfma_bf16(afr, bfr[nt], acc[r][t][nt]);
+        for (int nt = 0; nt < NT; ++nt) acc[r][t][nt] = mfma_bf16(bfr[nt], afr, acc[r][t][nt]);
       }
   }
+  // epilogue: lane holds channels nt*16 + 4*g + i of pixel x0 + t*16 + m
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int y = y0 + wave * RPW + r;
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t) {
+      const int x = x0 + t * 16 + m;
+      const size_t p = img + (size_t)y * W + x;
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const int n = nt * 16 + m;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int x = x0 + t * 16 + 4 * g + i;
-          const size_t p = img + (size_t)y * W + x;
-          const float s = acc[r][t][nt][i];
-          if constexpr (EPI == EPI_HIDDEN) {
-            const float z = s + bias[n];
-            zout[p * CO + n] = f2bf_dev(z);
-            aout[p * CO + n] = f2bf_dev(gelu(z));
-          } else if constexpr (EPI == EPI_INFER) {
-            aout[p * CO + n] = f2bf_dev(gelu(s + bias[n]));
-          } else if constexpr (EPI == EPI_OUT) {
-            if (n < 2) {
-              const size_t q = ((size_t)b * 2 + n) * H * W + (size_t)y * W + x;
-              state_out[q] = state_in[q] + scale[n] * (s + bias[n]);
-            }
-          } else {
-            aout[p * CO + n] = f2bf_dev(s * gelu_grad(bf2f(zq[p * CO + n])));
-          }
-        }
-      }
+      for (int nt = 0; nt < NT; ++nt)
+        epilogue4<EPI>(acc[r][t][nt], b, y, x, p, nt * 16 + 4 * g, H, W, CO, zout, aout, zq, state_in, state_out,
+                       scale);
+    }
   }
 }
 
@@ -675,7 +718,8 @@ int Sur_Num_Params(void) {
 
 // H, W: image size (W a multiple of 64, H a multiple of 16); maxB: largest batch
 int Sur_Init(int H, int W, int maxB) {
-  if (net.ready || W % 64 != 0 || H % 16 != 0 || maxB < 1) return -1;
+  // the convolutions address one image with 32-bit element offsets
+  if (net.ready || W % 64 != 0 || H % 16 != 0 || maxB < 1 || (long long)H * W * 48 >= (1LL << 31)) return -1;
   net.H = H;
   net.W = W;
   net.maxB = maxB;

@@ -22,7 +22,7 @@ extern "C" {
 /* number of FP32 parameters (weights and biases of the five layers) */
 int Sur_Num_Params(void);
 /* allocate the network for images H x W (W a multiple of 64, H a multiple of
- * 16) and batches of up to maxB instances */
+ * 16, H * W * 48 < 2^31) and batches of up to maxB instances */
 int Sur_Init(int H, int W, int maxB);
 /* parameters and gradients, host arrays of Sur_Num_Params() floats */
 int Sur_Set_Params(const float *host);
