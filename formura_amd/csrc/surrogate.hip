@@ -592,8 +592,8 @@ void cast_all() {
 }
 
 // convolution kernel: 0 = fragments from global memory (conv_mfma), 1 = LDS
-// tile of 4 rows (default), 2 = LDS tile of 8 rows (conv_lds; padded pixels)
-static int conv_kind = 1;
+// tile of 4 rows, 2 = LDS tile of 8 rows (conv_lds; padded pixels; default)
+static int conv_kind = 2;
 
 template <int CI, int NT, int DIR, int EPI>
 void launch_conv(const bf16* in, const bf16* wt, const float* bias, bf16* z, bf16* a, const bf16* zq,

@@ -75,6 +75,8 @@ class Surrogate:
     """The CNN library (one per process; its weights and buffers are global)."""
 
     _instance: Optional["Surrogate"] = None
+    CONV_KINDS = 3
+    DEFAULT_CONV = 2
 
     def __init__(self, max_batch: int = 16):
         path = os.path.join(_build.BUILD, "libsurrogate.so")
@@ -159,8 +161,9 @@ class Surrogate:
         return [float(x) for x in out]
 
     def set_conv(self, kind: int) -> None:
-        """Convolution kernel: 0 fragments from global memory, 1 LDS tile of
-        4 rows (default), 2 LDS tile of 8 rows; the same bits."""
+        """Convolution kernel (0 .. CONV_KINDS - 1; all give the same bits):
+        0 fragments from global memory, 1 / 2 LDS tile of 4 / 8 rows.
+        DEFAULT_CONV is the library's default."""
         if self.lib.Sur_Set_Conv(int(kind)) != 0:
             raise ValueError("bad convolution kernel")
 
@@ -600,7 +603,8 @@ def main(argv=None):
     import json
     ap = argparse.ArgumentParser(prog="python -m formura_amd.surrogate")
     ap.add_argument("command", choices=["train", "evaluate", "time-forward"])
-    ap.add_argument("--conv", type=int, default=1, help="time-forward: convolution kernel (0, 1, 2)")
+    ap.add_argument("--conv", type=int, default=Surrogate.DEFAULT_CONV,
+                    help=f"time-forward: convolution kernel (0 .. {Surrogate.CONV_KINDS - 1})")
     ap.add_argument("--reps", type=int, default=20, help="time-forward: timed forward passes")
     ap.add_argument("--seconds", type=float, default=240.0)
     ap.add_argument("--trajectories", type=int, default=2000)

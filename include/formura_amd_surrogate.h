@@ -30,8 +30,8 @@ int Sur_Get_Params(float *host);
 int Sur_Get_Grads(float *host);
 /* per-channel output scale s (2 floats, host): out = state + s * z5 */
 int Sur_Set_Scale(const float *s2);
-/* convolution kernel: 0 fragments from global memory, 1 LDS tile of 4 rows
- * (default), 2 LDS tile of 8 rows; all give the same bits */
+/* convolution kernel (all give the same bits): 0 fragments from global
+ * memory, 1 LDS tile of 4 rows, 2 LDS tile of 8 rows (default) */
 int Sur_Set_Conv(int kind);
 /* one pass (32 steps) for B instances: out = net(state, fk) */
 int Sur_Forward(const float *state, const float *fk, float *out, int B);

@@ -75,6 +75,45 @@ def test_forward_and_gradients_match_oracle():
         b.free()
 
 
+def test_convolution_kernels_give_the_same_bits():
+    """Every convolution kernel (Sur_Set_Conv) forms each output's sum over
+    the same k-steps in the same order, so the inference forward pass, the
+    training forward pass (pre-activations kept), the loss and every gradient
+    (data gradients through the same kernels) are bit-identical across them."""
+    from formura_amd import surrogate as S
+    sur = S.Surrogate.get(16)
+    specs, state, target, fk = _case(B=3, seed=5)
+    B = len(specs)
+    sur.set_params(S.init_params(13))
+    sur.set_scale(0.02, 0.01)
+    bs, bt, bf, bo = (sur.buffer(state.nbytes), sur.buffer(target.nbytes), sur.buffer(fk.nbytes),
+                      sur.buffer(state.nbytes))
+    bs.upload(state)
+    bt.upload(target)
+    bf.upload(fk)
+    got = {}
+    try:
+        for kind in range(S.Surrogate.CONV_KINDS):
+            _say(f"[surrogate] convolution kernel {kind}")
+            sur.set_conv(kind)
+            sur.forward(bs, bf, bo, B)
+            out = bo.download(state.shape)
+            loss = sur.gradients(bs, bf, bt, B)
+            got[kind] = (out, loss, sur.get_grads())
+    finally:
+        sur.set_conv(S.Surrogate.DEFAULT_CONV)
+    out0, loss0, g0 = got[0]
+    assert np.isfinite(out0).all() and np.isfinite(g0).all()
+    for kind, (out, loss, g) in got.items():
+        assert np.array_equal(out.view(np.uint32), out0.view(np.uint32)), f"kernel {kind}: forward"
+        assert loss == loss0, f"kernel {kind}: loss"
+        assert np.array_equal(g.view(np.uint32), g0.view(np.uint32)), f"kernel {kind}: gradients"
+    with pytest.raises(ValueError):
+        sur.set_conv(S.Surrogate.CONV_KINDS)
+    for b in (bs, bt, bf, bo):
+        b.free()
+
+
 def test_training_reduces_loss():
     """A short training run on the GPU (its own forward, backward and Adam
     kernels) lowers the loss on held-out stencil pairs -- trajectories with
