@@ -334,7 +334,8 @@ class StepLibrary {
   void launch_march(const VariantEntry<P>* e) {
     const int T = e->v.T, rows = e->v.by * e->v.waves;
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
-    const int ntz = (n[2] + (WAVE - 2 * T) - 1) / (WAVE - 2 * T);
+    const int zw = T == 1 ? WAVE : WAVE - 2 * T;   // z sites stored per wave (T = 1: no ghost lanes)
+    const int ntz = (n[2] + zw - 1) / zw;
     int groups = 0, chunk = 0;
     march_split(nty, ntz, e->v.waves, groups, chunk);
     e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], groups, nty, ntz, chunk, prm, stream);

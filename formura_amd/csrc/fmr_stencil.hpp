@@ -80,6 +80,16 @@ __device__ __forceinline__ double lane_next(double v) {
   const int hi = dpp_shl1((int)(q >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// the value of lane l (uniform l) in every lane
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long q = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(q & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(q >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // ---- buffer loads and stores (raw buffer: uniform base in a resource, 32-bit
 // per-lane byte offset).  A resource spans one plane of a field, so the base
@@ -279,7 +289,11 @@ __global__ void __launch_bounds__(64 * ROWS) stage_kernel(Arrays<typename S::rea
 // Ghost zones: the tile carries T ghost rows / lanes on each side, recomputed
 // redundantly; after level l the valid part is rows [l, W*BY-l) and lanes
 // [l, 64-l).  Only level T's valid part is written.  Reads wrap periodically,
-// so the ghost zones also implement the periodic boundary.  The window slot
+// so the ghost zones also implement the periodic boundary.  With T = 1 there
+// are no ghost lanes: a wave's 64 lanes are 64 consecutive z sites from a
+// multiple of 64, so every row it loads or stores is whole, aligned cache
+// lines, and the two sites just outside its z range come from one extra
+// load per plane (the z neighbours of lanes 0 and 63).  The window slot
 // of the newest plane rotates with the iteration phase (0,1,2), so the march
 // loop is unrolled by 3 and no register moves are needed.
 //
@@ -312,13 +326,37 @@ struct March3D {
   static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
   using Edge = real[2][T][W][2][NF][WAVE];   // [half][level][wave][top,bottom][field][lane]
 
+  // T = 1: a wave's 64 lanes are 64 consecutive z sites from a multiple of 64
+  // (whole, aligned cache lines; no ghost lanes).  The z neighbours of lanes
+  // 0 and 63 come from a halo load per plane: lane l of halo register j holds
+  // row 32j + l/2's site just before (l even) or just after (l odd) the
+  // wave's z range.  T > 1: 2T ghost lanes, whose sites other tiles store.
+  static constexpr bool ZFULL = (T == 1);
+  static constexpr int NH = ZFULL ? (BY + 31) / 32 : 1;
+
   real w[T][3][BY][NF];
   real nxt[PF][BY][NF];   // planes prefetched PF iterations ahead
+  real hw[3][NH][NF];     // halo registers of the window planes (ZFULL)
+  real hnxt[PF][NH][NF];  // and of the prefetched planes
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
   unsigned zoff;       // byte offset of this lane's z-column within a row
   unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane (the store is dropped)
   unsigned roff[BY];   // byte offset of row r within a plane (uniform: scalar registers)
+  unsigned hoff[NH];   // byte offset of this lane's halo site within a plane (ZFULL)
+
+  // z neighbours of a row of plane phase CEN at level 0 (ZFULL): the lane
+  // shift, with the halo site in lane 0 (before) or lane 63 (after)
+  template <int CEN, int ROW, int F>
+  __device__ __forceinline__ real z_before(real shifted) const {
+    const real h = read_lane(hw[CEN][ROW / 32][F], 2 * (ROW % 32));
+    return lane == 0 ? h : shifted;
+  }
+  template <int CEN, int ROW, int F>
+  __device__ __forceinline__ real z_after(real shifted) const {
+    const real h = read_lane(hw[CEN][ROW / 32][F], 2 * (ROW % 32) + 1);
+    return lane == WAVE - 1 ? h : shifted;
+  }
 
   template <int L, int CEN, int ROW, int F>
   __device__ __forceinline__ real row_above() const {
@@ -348,17 +386,33 @@ struct March3D {
       else if constexpr (D0 == 1) return m.w[L][NEW][ROW][F];
       else if constexpr (D1 == -1) return m.template row_above<L, CEN, ROW, F>();
       else if constexpr (D1 == 1) return m.template row_below<L, CEN, ROW, F>();
-      else if constexpr (D2 == -1) return lane_prev(m.w[L][CEN][ROW][F]);
-      else if constexpr (D2 == 1) return lane_next(m.w[L][CEN][ROW][F]);
-      else return m.w[L][CEN][ROW][F];
+      else if constexpr (D2 == -1) {
+        if constexpr (ZFULL) return m.template z_before<CEN, ROW, F>(lane_prev(m.w[L][CEN][ROW][F]));
+        else return lane_prev(m.w[L][CEN][ROW][F]);
+      } else if constexpr (D2 == 1) {
+        if constexpr (ZFULL) return m.template z_after<CEN, ROW, F>(lane_next(m.w[L][CEN][ROW][F]));
+        else return lane_next(m.w[L][CEN][ROW][F]);
+      } else {
+        return m.w[L][CEN][ROW][F];
+      }
     }
     __device__ __forceinline__ double param(int i) const { return prm.v[i]; }
     __device__ __forceinline__ int index(int axis) const { return axis == 0 ? cx : (axis == 1 ? cy : cz); }
     template <int F>
     __device__ __forceinline__ void st(real v) { out[F] = v; }
-    // the lane axis is z: the previous site's value is in the previous lane
+    // the lane axis is z: the previous site's value is in the previous lane;
+    // lane 0 of a ZFULL wave has none there and evaluates it (the same
+    // operations on the same values: the same bits)
     template <class V, class Fn>
-    __device__ __forceinline__ V lane_prev_of(V v, Fn) const { return lane_prev(v); }
+    __device__ __forceinline__ V lane_prev_of(V v, Fn fallback) const {
+      if constexpr (ZFULL) {
+        const V shifted = lane_prev(v);
+        const V own = fallback();
+        return m.lane == 0 ? own : shifted;
+      } else {
+        return lane_prev(v);
+      }
+    }
   };
 
   // load plane x (already wrapped) into prefetch slot K
@@ -369,6 +423,10 @@ struct March3D {
       const auto rs = plane_rsrc(reinterpret_cast<const char*>(in.f[f]) + (size_t)x * plane_bytes, plane_bytes);
 #pragma unroll
       for (int r = 0; r < BY; ++r) nxt[K][r][f] = buf_load((real*)nullptr, rs, zoff, roff[r]);
+      if constexpr (ZFULL) {
+#pragma unroll
+        for (int j = 0; j < NH; ++j) hnxt[K][j][f] = buf_load((real*)nullptr, rs, hoff[j]);
+      }
     }
   }
 
@@ -447,6 +505,16 @@ struct March3D {
 #pragma unroll
         for (int k = 0; k + 1 < PF; ++k) nxt[k][r][f] = nxt[k + 1][r][f];
       }
+    if constexpr (ZFULL) {
+#pragma unroll
+      for (int j = 0; j < NH; ++j)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          hw[PH][j][f] = hnxt[0][j][f];
+#pragma unroll
+          for (int k = 0; k + 1 < PF; ++k) hnxt[k][j][f] = hnxt[k + 1][j][f];
+        }
+    }
     if constexpr (W > 1) {
       // publish this wave's edge rows of every window centre, then one barrier
 #pragma unroll
@@ -485,7 +553,8 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
   const int tz = tile % ntz;
   const int ty = tile / ntz;
   const int y0 = ty * (M::ROWS - 2 * H) - H;          // tile row 0
-  const int zu = tz * (WAVE - 2 * H) - H + m.lane;     // unwrapped z of this lane
+  const int z0 = M::ZFULL ? tz * WAVE : tz * (WAVE - 2 * H) - H;   // unwrapped z of lane 0
+  const int zu = z0 + m.lane;                           // unwrapped z of this lane
   const int zg = wrap(zu, nz);
 
   const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
@@ -493,7 +562,18 @@ __device__ __forceinline__ void march_run(March3D<P, T, BY, W, PF>& m, const Fie
   for (int r = 0; r < BY; ++r)
     m.roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + m.wave * BY + r, ny) * nz * (int)sizeof(real));
   m.zoff = (unsigned)(zg * sizeof(real));
-  m.zst = (m.lane >= H && m.lane < WAVE - H && zu < nz) ? m.zoff : BUF_OOB;   // lanes past the grid end: duplicates, not written
+  // lanes past the grid end hold periodic duplicates and are not written
+  if constexpr (M::ZFULL) {
+    m.zst = zu < nz ? m.zoff : BUF_OOB;
+    const int zh = wrap((m.lane & 1) ? z0 + WAVE : z0 - 1, nz);
+#pragma unroll
+    for (int j = 0; j < M::NH; ++j) {
+      const int r = min(32 * j + m.lane / 2, BY - 1);
+      m.hoff[j] = (unsigned)((wrap(y0 + m.wave * BY + r, ny) * nz + zh) * (int)sizeof(real));
+    }
+  } else {
+    m.zst = (m.lane >= H && m.lane < WAVE - H && zu < nz) ? m.zoff : BUF_OOB;
+  }
   int xl = wrap(x0 - H, nx);
   m.template load_plane<0>(in, xl, plane_bytes);
   if constexpr (PF > 1) {
