@@ -8,7 +8,8 @@ step library.  A "step" is one time step over the whole grid; `value` is
 lattice-site updates per second (GLUP/s) with the state resident in HBM.
 The kernel variant (fusion depth T, tile shape, work split) is chosen by a
 short timed search before the timed region; the best rate for each compiled
-fusion depth T is reported as well (t_sweep_glups).
+fusion depth T is reported as well (t_sweep_glups), with its fraction of the
+HBM roofline (t_sweep_frac: 16 / T bytes per update).
 
 Also reported (same line, key "extra"): the FP32 bulk heat rate, 3D Gray-Scott
 512^3 in FP64 and FP32, 3D Gray-Scott 64^3 FP64 (1000 steps), the multi-stage
@@ -312,6 +313,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "t_sweep_glups": sweep,
+        # fraction of the 8 TB/s roofline per fusion depth: a T-fused launch
+        # reads and writes the FP64 state once, 16 / T bytes per update
+        "t_sweep_frac": {T: round(r * 16 / T / HBM_PEAK_GBS, 4) for T, r in sweep.items()},   # GLUP/s x B = GB/s
         "kernel_ms_per_launch": round(launch_ms, 5),
         "kernel_launches": launches,
         "gpu_event_ms": round(ev_ms, 4),
