@@ -59,7 +59,9 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
       * one wave per tile at two waves per SIMD, prefetching one and two
         planes ahead (and, for T <= 2, at four waves per SIMD);
       * a tile of up to 64 + 2T rows split over up to 4 waves that exchange
-        their edge rows through LDS."""
+        their edge rows through LDS (one wave per SIMD; for T = 1, where the
+        re-read ghost rows are the kernel's main excess traffic, at two waves
+        per SIMD)."""
     words = 2 if real == "double" else 1
     if max_t is None:
         # FP64: 1..5, and 8 for the sweep over {1, 2, 4, 8} (only a
@@ -77,6 +79,12 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
             by = min(_max_rows(T, nf, words, pf, budget), 64 + 2 * T)
             if by >= 2 * T + 2:
                 cand.append((T, by, 1, pf, occ))
+        if T == 1:
+            by2 = _max_rows(T, nf, words, 1, OCC2_REG_BUDGET)
+            rows = min(64 + 2 * T, by2 * MAX_WAVES)
+            W = -(-rows // by2)
+            if W > 1:
+                cand.append((T, -(-rows // W), W, 1, 2))
         by_max = _max_rows(T, nf, words, 1, AGPR_REG_BUDGET)
         if T >= 2 and by_max * MAX_WAVES >= 2 * T + 2:
             rows = min(64 + 2 * T, by_max * MAX_WAVES)
