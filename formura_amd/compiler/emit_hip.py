@@ -51,7 +51,7 @@ def _max_rows(T: int, nf: int, words: int, pf: int, budget: int) -> int:
     return by
 
 
-def default_march_variants(nf: int, real: str, max_t: int = None):
+def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None):
     """(T, rows per wave, waves, prefetch depth, waves per SIMD) variants of
     the march kernel, for each fusion depth T:
       * one wave per tile (no LDS, no barrier) with as many rows as fit the
@@ -61,7 +61,10 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
       * a tile of up to 64 + 2T rows split over up to 4 waves that exchange
         their edge rows through LDS (one wave per SIMD; for T = 1, where the
         re-read ghost rows are the kernel's main excess traffic, at two waves
-        per SIMD)."""
+        per SIMD);
+      * given the grid's y extent ny, for T >= 2 also the multi-wave tile
+        whose stored rows (W * rows - 2T) are the largest divisor of ny that
+        fits: no tile then computes and stores rows another tile owns."""
     words = 2 if real == "double" else 1
     if max_t is None:
         # FP64: 1..5, and 8 for the sweep over {1, 2, 4, 8} (only a
@@ -91,10 +94,23 @@ def default_march_variants(nf: int, real: str, max_t: int = None):
             W = -(-rows // by_max)
             if W > 1:
                 cand.append((T, -(-rows // W), W, 1, 1))
+        if T >= 2 and ny:
+            # (stored rows, rows per wave, waves): stored rows divide ny and
+            # are at least 4T (ghost rows at most a third of the tile)
+            fits = [(W * by - 2 * T, by, W) for W in range(2, MAX_WAVES + 1) for by in range(1, by_max + 1)
+                    if W * by - 2 * T >= 4 * T and W * by <= 64 + 2 * T and ny % (W * by - 2 * T) == 0]
+            if fits:
+                _, by, W = max(fits)
+                cand.append((T, by, W, 1, 1))
         for v in cand:
             if v not in out:
                 out.append(v)
     return out
+
+
+def _march_ny(lay: Layout):
+    """The y extent the march kernel tiles (3D programs), else None."""
+    return lay.intra[1] if len(lay.intra) == 3 else None
 
 
 def device_real(lay: Layout, precision: str) -> str:
@@ -103,7 +119,7 @@ def device_real(lay: Layout, precision: str) -> str:
 
 def variants_for(lay: Layout, precision: str):
     """The march variants emit_hip compiles in by default."""
-    return default_march_variants(len(lay.field_names), device_real(lay, precision))
+    return default_march_variants(len(lay.field_names), device_real(lay, precision), ny=_march_ny(lay))
 
 
 def lane_reuse(ir: PointIR, real: str):
@@ -196,7 +212,7 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     inlined = len(ir.ops) <= INLINE_LIMIT or not has_stages
     use_march = dim == 3 and star and max(r) <= 1 and inlined
     if march_variants is None:
-        march_variants = default_march_variants(len(lay.field_names), real)
+        march_variants = default_march_variants(len(lay.field_names), real, ny=_march_ny(lay))
     variants = list(march_variants) if use_march else []
     n = list(lay.intra) + [1] * (3 - dim)
     shape = lay.shape() + [1] * (3 - dim)
