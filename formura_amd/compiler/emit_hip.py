@@ -62,9 +62,11 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
         their edge rows through LDS (one wave per SIMD; for T = 1, where the
         re-read ghost rows are the kernel's main excess traffic, at two waves
         per SIMD);
-      * given the grid's y extent ny, for T >= 2 also the multi-wave tile
-        whose stored rows (W * rows - 2T) are the largest divisor of ny that
-        fits: no tile then computes and stores rows another tile owns."""
+      * for T >= 2 also a multi-wave tile at two waves per SIMD;
+      * given the grid's y extent ny, for T >= 2 also the multi-wave tiles
+        (one and two waves per SIMD) whose stored rows (W * rows - 2T) are
+        the largest divisor of ny that fits: no tile then computes and
+        stores rows another tile owns."""
     words = 2 if real == "double" else 1
     if max_t is None:
         # FP64: 1..5, and 8 for the sweep over {1, 2, 4, 8} (only a
@@ -94,14 +96,21 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
             W = -(-rows // by_max)
             if W > 1:
                 cand.append((T, -(-rows // W), W, 1, 1))
+        by2 = _max_rows(T, nf, words, 1, OCC2_REG_BUDGET)
+        if T >= 2 and MAX_WAVES * by2 - 2 * T >= 4 * T:
+            # the same at two waves per SIMD: fewer rows, no AGPR traffic
+            rows = min(64 + 2 * T, by2 * MAX_WAVES)
+            W = -(-rows // by2)
+            cand.append((T, -(-rows // W), W, 1, 2))
         if T >= 2 and ny:
             # (stored rows, rows per wave, waves): stored rows divide ny and
             # are at least 4T (ghost rows at most a third of the tile)
-            fits = [(W * by - 2 * T, by, W) for W in range(2, MAX_WAVES + 1) for by in range(1, by_max + 1)
-                    if W * by - 2 * T >= 4 * T and W * by <= 64 + 2 * T and ny % (W * by - 2 * T) == 0]
-            if fits:
-                _, by, W = max(fits)
-                cand.append((T, by, W, 1, 1))
+            for budget_rows, occ in ((by_max, 1), (by2, 2)):
+                fits = [(W * by - 2 * T, by, W) for W in range(2, MAX_WAVES + 1) for by in range(1, budget_rows + 1)
+                        if W * by - 2 * T >= 4 * T and W * by <= 64 + 2 * T and ny % (W * by - 2 * T) == 0]
+                if fits:
+                    _, by, W = max(fits)
+                    cand.append((T, by, W, 1, occ))
         for v in cand:
             if v not in out:
                 out.append(v)
