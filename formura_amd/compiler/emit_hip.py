@@ -38,6 +38,12 @@ INLINE_LIMIT = 6000
 OCC2_REG_BUDGET = 236    # two waves per SIMD (256 registers each, some slack for the scheduler)
 OCC4_REG_BUDGET = 112    # four waves per SIMD (128 registers each)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
+MAX_EDGE_LDS = 96 * 1024   # LDS of a multi-wave tile's edge-row exchange (160 KB per CU)
+
+
+def edge_lds_bytes(T: int, W: int, nf: int, words: int) -> int:
+    """LDS of the march kernel's edge exchange: [2][T][W][2][NF][64] values."""
+    return 2 * T * W * 2 * nf * 64 * 4 * words
 
 
 def march_regs(T: int, by: int, nf: int, words: int, pf: int = 1) -> int:
@@ -62,7 +68,9 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
         their edge rows through LDS (one wave per SIMD; for T = 1, where the
         re-read ghost rows are the kernel's main excess traffic, at two waves
         per SIMD);
-      * for T >= 2 also a multi-wave tile at two waves per SIMD;
+      * for T >= 2 also multi-wave tiles at two waves per SIMD: up to 4
+        waves, and (T <= 5) 8 waves, a whole CU, with two rows per wave
+        below the register limit;
       * given the grid's y extent ny, for T >= 2 also the multi-wave tiles
         (one and two waves per SIMD) whose stored rows (W * rows - 2T) are
         the largest divisor of ny that fits: no tile then computes and
@@ -102,6 +110,11 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
             rows = min(64 + 2 * T, by2 * MAX_WAVES)
             W = -(-rows // by2)
             cand.append((T, -(-rows // W), W, 1, 2))
+        by8 = by2 - 2     # register slack for the scheduler
+        if 2 <= T <= 5 and 8 * by8 - 2 * T >= 4 * T and edge_lds_bytes(T, 8, nf, words) <= MAX_EDGE_LDS:
+            # a whole workgroup of 8 waves per CU (two per SIMD): ghost rows
+            # shared by twice as many stored rows
+            cand.append((T, by8, 8, 1, 2))
         if T >= 2 and ny:
             # (stored rows, rows per wave, waves): stored rows divide ny and
             # are at least 4T (ghost rows at most a third of the tile)

@@ -366,15 +366,18 @@ class StepLibrary {
     return k;
   }
 
-  // `left` < T steps: the deepest variant with the same tile shape that fits,
-  // else any fused variant that fits, else the direct kernel
+  // `left` < T steps: a variant of the deepest fusion depth that fits (the
+  // selected tile shape if it is compiled at that depth, else the first
+  // variant of that depth), else the direct kernel
   void advance_rest(int left) {
     while (left > 0) {
       const VariantEntry<P>* e = nullptr;
-      for (int t = left < choice.T ? left : choice.T; t >= 1 && !e; --t)
-        e = find(KIND_MARCH, t, choice.by, choice.waves, choice.pf, choice.occ);
+      int depth = 0;
+      for (int i = 0; i < nvariants; ++i)
+        if (variants[i].v.T <= left && variants[i].v.T > depth) depth = variants[i].v.T;
+      if (depth > 0) e = find(KIND_MARCH, depth, choice.by, choice.waves, choice.pf, choice.occ);
       for (int i = 0; i < nvariants && !e; ++i)
-        if (variants[i].v.T <= left) e = &variants[i];
+        if (variants[i].v.T == depth) e = &variants[i];
       if (!e) {
         launch_direct(left);
         return;

@@ -239,11 +239,14 @@ def extra_oracle(name):
     return _cache[key]
 
 
-@pytest.mark.parametrize("variant", [(KIND_MARCH, 3, 17, 1, 1, 1, 32), (KIND_MARCH, 4, 14, 4, 1, 1, -1)])
+@pytest.mark.parametrize("variant", [(KIND_MARCH, 3, 17, 1, 1, 1, 32), (KIND_MARCH, 4, 14, 4, 1, 1, -1),
+                                     (KIND_MARCH, 4, 8, 4, 1, 2, 64)])
 def test_heat512_full_size_bitwise(variant):
     """The headline workload at its full size: heat 512^3 FP64, 10 steps of
-    the march variants the benchmark's search picks (T=3 and T=4, so 3+3+3+1
-    and 4+4+1+1 launches), against the oracle's 10 steps, bit for bit."""
+    march variants the benchmark's search has picked (T=3; T=4 at one and at
+    two waves per SIMD), against the oracle's 10 steps, bit for bit.  The
+    steps left over run on the deepest variant that fits: 3+3+3+1 and 4+4+2
+    launches."""
     lib = hip("heat512")
     o = extra_oracle("heat512_cpu")
     assert o.layout.steps_per_forward == 10
