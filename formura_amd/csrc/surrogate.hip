@@ -882,6 +882,17 @@ int Sur_Gather(const unsigned long long* host_ptrs, float* dst, int nptr, size_t
   return 0;
 }
 
+// *ptrs[i] = src[i * n_each .. (i+1) * n_each) for i < nptr: the inverse of
+// Sur_Gather, asynchronous copies on the library stream and one wait at the end
+int Sur_Scatter(const unsigned long long* host_ptrs, const float* src, int nptr, size_t n_each) {
+  if (nptr < 0) return -1;
+  for (int i = 0; i < nptr; ++i)
+    SCHECK(hipMemcpyAsync((void*)host_ptrs[i], src + (size_t)i * n_each, n_each * sizeof(float),
+                          hipMemcpyDeviceToDevice, net.stream));
+  SCHECK(hipStreamSynchronize(net.stream));
+  return 0;
+}
+
 int Sur_Add_Noise(float* x, size_t n, float sigma, unsigned long long seed) {
   hipLaunchKernelGGL(add_noise, dim3(grid_for(n, 256)), dim3(256), 0, net.stream, x, n, sigma, seed);
   SCHECK(hipGetLastError());

@@ -46,6 +46,9 @@ int Sur_Free(void *p);
 int Sur_Copy(void *dst, const void *src, size_t bytes, int kind);
 /* gather nptr device arrays of n_each floats (addresses in host_ptrs) into dst */
 int Sur_Gather(const unsigned long long *host_ptrs, float *dst, int nptr, size_t n_each);
+/* the inverse: scatter src into nptr device arrays of n_each floats; returns
+ * after the copies are done (they are visible to any stream) */
+int Sur_Scatter(const unsigned long long *host_ptrs, const float *src, int nptr, size_t n_each);
 /* x += N(0, sigma^2) noise (counter-based, from seed) */
 int Sur_Add_Noise(float *x, size_t n, float sigma, unsigned long long seed);
 /* relative L2 distance per instance of B arrays of n_each floats (device
