@@ -85,13 +85,16 @@ def staged_bytes_per_site(name):
 
 
 def candidates(lib):
-    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED
+    """Every (kernel, work split) the search may pick: the direct kernel, the
+    stage kernels, and each compiled march variant with each split of
+    runtime.march_chunks -- the set the full-size parity tests run."""
+    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED, march_chunks
     out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)] if has_direct(lib) else []
     if lib.num_stages() > 0:
         for rows in (4, 16):     # wavefronts per workgroup of the stage kernels
             out.append(((KIND_STAGED, 1, rows, 1, 1, 1), 64))
     for v in lib.list_kernels():
-        for lx in (32, 64, 128, -1, -2) + ((-4,) if v[4] >= 4 else ()):   # run length, or -workgroups per SIMD
+        for lx in march_chunks(v[4]):   # run length, or -workgroups per SIMD
             out.append(((KIND_MARCH,) + tuple(v), lx))
     return out
 
@@ -265,6 +268,10 @@ def main():
     if a.variant:
         v = [int(x) for x in a.variant.split(",")]
         var, lx = tuple(v[:6]), v[6]
+        if (var, lx) not in candidates(lib):
+            print(f"bench.py: variant {a.variant} is not one the parity tests run "
+                  f"(a compiled march variant with a split of runtime.march_chunks)", file=sys.stderr)
+            sys.exit(2)
     else:
         log("autotuning heat512 FP64 ...")
         best = autotune(lib)

@@ -14,7 +14,7 @@ from __future__ import annotations
 from typing import List, Sequence, Tuple
 
 from .abi import Layout
-from .emit_hip import is_star, point_body
+from .emit_hip import MAX_FIELDS, is_star, point_body
 from .pointir import PointIR
 
 # (T, points per lane) variants; every T has variants for every lane width so
@@ -43,6 +43,8 @@ def emit_batched(lay: Layout, ir: PointIR, precision: str = "fp32",
         raise ValueError("batched launches need a radius-1 star stencil")
     real = "float" if precision == "fp32" else lay.field_ctype[0]
     nf = len(lay.field_names)
+    if nf > MAX_FIELDS:
+        raise ValueError(f"{nf} state fields: the batched library holds at most {MAX_FIELDS}")
     if variants is None:
         variants = default_batch_variants(nf, real)
     out: List[str] = []

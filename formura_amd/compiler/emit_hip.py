@@ -39,6 +39,7 @@ OCC2_REG_BUDGET = 236    # two waves per SIMD (256 registers each, some slack fo
 OCC4_REG_BUDGET = 112    # four waves per SIMD (128 registers each)
 MAX_WAVES = 4            # <= 4 waves per workgroup keeps one wave per SIMD
 MAX_EDGE_LDS = 96 * 1024   # LDS of a multi-wave tile's edge-row exchange (160 KB per CU)
+MAX_FIELDS = 16            # state fields per program (fmr::MAX_FIELDS in csrc/fmr_stencil.hpp)
 
 
 def edge_lds_bytes(T: int, W: int, nf: int, words: int) -> int:
@@ -223,6 +224,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     if dim not in (2, 3):
         raise ValueError("the HIP backend supports 2D and 3D programs")
     real = device_real(lay, precision)
+    if len(lay.field_names) > MAX_FIELDS:
+        raise ValueError(f"{len(lay.field_names)} state fields: the HIP library holds at most {MAX_FIELDS}")
     if precision != "fp32" and len(set(lay.field_ctype)) != 1:
         raise ValueError("all state variables must share one element type")
     r = ir.radius() + [0] * (3 - dim)

@@ -170,8 +170,13 @@ class StepLibrary:
     def diff(self, f: int, ref: "StepLibrary"):
         """(L2 of difference, L2 of ref, max abs difference) of field f against
         the same field of another library's device state."""
+        return self.diff_device(f, ref.device_field(f), ref.elem_size())
+
+    def diff_device(self, f: int, ptr: int, elem_size: int):
+        """(L2 of difference, L2 of ref, max abs difference) of field f against
+        a device array at `ptr` of the same shape (elements of 8 or 4 bytes)."""
         r = (ctypes.c_double * 3)()
-        if self.lib.Formura_Diff_Device(f, ctypes.c_void_p(ref.device_field(f)), ref.elem_size(), r) != 0:
+        if self.lib.Formura_Diff_Device(f, ctypes.c_void_p(ptr), int(elem_size), r) != 0:
             raise ValueError("bad diff arguments")
         return float(np.sqrt(r[0])), float(np.sqrt(r[1])), float(r[2])
 
@@ -179,6 +184,15 @@ class StepLibrary:
 KIND_DIRECT = 0
 KIND_MARCH = 1
 KIND_STAGED = 2      # multi-stage programs: one kernel per manifest node
+
+
+def march_chunks(occ: int) -> tuple:
+    """The work splits (Formura_Set_March_Chunk values) offered for a march
+    variant compiled for `occ` waves per SIMD: runs of 32, 64 or 128 planes,
+    or x whole with one or two (four, for occ >= 4) workgroups per SIMD.  The
+    benchmark's search chooses among exactly these, and the full-size parity
+    tests run every compiled variant with each of them."""
+    return (32, 64, 128, -1, -2) + ((-4,) if occ >= 4 else ())
 
 
 def load_program(name: str) -> StepLibrary:

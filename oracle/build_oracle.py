@@ -36,8 +36,8 @@ _GS2D_PARAMS = ("F", "k", "dt", "cu", "cv")
 ORACLE_ONLY = {
     # the 512^3 heat program with 10 steps per Formura_Forward: the CPU-baseline sample
     "heat512_cpu": ("heat512", {"monitor_interval": 10}, ()),
-    # Gray-Scott 512^3 with 4 steps per Formura_Forward: full-size GPU parity
-    "gs3d512_cpu": ("gs3d512", {"monitor_interval": 4}, ()),
+    # Gray-Scott 512^3 with 10 steps per Formura_Forward: full-size GPU parity
+    "gs3d512_cpu": ("gs3d512", {"monitor_interval": 10}, ()),
     # 2D Gray-Scott with per-run parameters, for the batched-launch parity tests
     "gs2d_p256": ("gs2d", {"monitor_interval": 40}, _GS2D_PARAMS),
     "gs2d_p96x160": ("gs2d", {"intra_node_shape": [96, 160], "monitor_interval": 40}, _GS2D_PARAMS),

@@ -78,3 +78,38 @@ def test_split_input_keeps_sixteen_bits():
     split = O.conv(s, O.tie(w))
     bound = 2.0 ** -16 * O.conv(np.abs(x), np.abs(w))
     assert np.all(np.abs(split - full) <= bound + 1e-15)
+
+
+def test_forward_matches_torch_float64():
+    """The oracle's unquantised forward pass against an independent statement
+    of the same network in torch (float64): circular padding by
+    torch.nn.functional.pad(mode="circular"), torch's conv2d (cross-
+    correlation: out[p] = sum_d W[d] a[p + d], the oracle's convention) and
+    the exact (erf) GELU.  A wrong tap offset, a transposed kernel or a
+    missing wrap would change the result far beyond the tolerance."""
+    torch = __import__("pytest").importorskip("torch")
+    F = torch.nn.functional
+    rng = np.random.default_rng(5)
+    B, H, W = 2, 7, 9          # odd and unequal sides: a wrong axis or shift shows
+    state = rng.uniform(0.0, 1.0, (B, 2, H, W))
+    fk = np.array([[0.03, 0.06], [0.05, 0.052]])
+    theta = rng.normal(0.0, 0.2, O.num_params())
+    scale = (0.3, 0.2)
+    got = O.forward(theta, state, fk, scale, quantize=False)
+
+    x = torch.from_numpy(O.pack_input(state, fk)).permute(0, 3, 1, 2)   # [B][4][H][W]
+    a = x
+    layers = O.split(theta)
+    for l, (w, b) in enumerate(layers):
+        wt = torch.from_numpy(np.ascontiguousarray(w.transpose(0, 3, 1, 2)))   # [co][ci][3][3]
+        z = F.conv2d(F.pad(a, (1, 1, 1, 1), mode="circular"), wt, torch.from_numpy(b))
+        a = F.gelu(z, approximate="none") if l < len(layers) - 1 else z
+    out = torch.from_numpy(state).clone()
+    for c in range(2):
+        out[:, c] += scale[c] * a[:, c]
+    ref = out.numpy()
+    inc_o, inc_t = got["out"] - state, ref - state
+    assert np.linalg.norm(inc_o - inc_t) <= 1e-12 * np.linalg.norm(inc_t)
+    # the last layer's pre-activation as well, channel by channel
+    z5 = a.permute(0, 2, 3, 1).numpy()
+    assert np.max(np.abs(got["z5"] - z5)) <= 1e-12 * np.max(np.abs(z5))
