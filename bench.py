@@ -88,14 +88,14 @@ def candidates(lib):
     """Every (kernel, work split) the search may pick: the direct kernel, the
     stage kernels, and each compiled march variant with each split of
     runtime.march_chunks -- the set the full-size parity tests run."""
-    from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, KIND_STAGED, march_chunks
+    from formura_amd.runtime import KIND_DIRECT, KIND_STAGED, march_chunks
     out = [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)] if has_direct(lib) else []
     if lib.num_stages() > 0:
         for rows in (4, 16):     # wavefronts per workgroup of the stage kernels
             out.append(((KIND_STAGED, 1, rows, 1, 1, 1), 64))
     for v in lib.list_kernels():
-        for lx in march_chunks(v[4]):   # run length, or -workgroups per SIMD
-            out.append(((KIND_MARCH,) + tuple(v), lx))
+        for lx in march_chunks(v[5]):   # run length, or -workgroups per SIMD
+            out.append((tuple(v), lx))
     return out
 
 
@@ -193,17 +193,21 @@ EXTRA_PROGRAMS = (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp6
                   ("fveuler", "fv_euler256_ssprk3_fp64"))
 
 
-def program_extras(only=None):
+def program_extras(only=None, libs=None):
     """Autotuned rate of each program of EXTRA_PROGRAMS (or only the one
-    labelled `only`); multi-stage programs also report their stage kernels."""
+    labelled `only`); multi-stage programs also report their stage kernels.
+    The loaded libraries are left in `libs` (name -> library), each set to
+    the kernel it was timed with."""
     import numpy as np
     from formura_amd.runtime import load_program
     extra = {}
+    libs = {} if libs is None else libs
     for name, label in EXTRA_PROGRAMS:
         if only and label != only:
             continue
         lg = load_program(name)
         lg.init()
+        libs[name] = lg
         lg.set_state(seeded_state(lg, 1))
         lg.upload()
         n = int(np.prod(lg.layout.intra))
@@ -233,8 +237,58 @@ def program_extras(only=None):
                         "rows": rows, "glups_by_rows": {r: round(20 * n / v[0] / 1e9, 2) for r, v in runs.items()},
                         "algorithmic_bytes_per_step": b, "ms_per_step": round(ev2 / 20, 4),
                         "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        lg.set_kernel(*v)                  # leave the library on the kernel it was timed with
+        lg.set_chunk(l)
         log(f"{label}: {extra[label]}")
     return extra
+
+
+def precision_errors(pairs, seed=2):
+    """FP32 bulk against its FP64 twin on the GPU, on the same seeded input:
+    for each (label, FP32 library, FP64 library, steps), both advance `steps`
+    steps with the kernels they were timed with, and every field is compared
+    on the device with FP64 reductions (relative L2 and max abs)."""
+    out = {}
+    for label, f32, f64, steps in pairs:
+        st = seeded_state(f64, seed)
+        for lib in (f32, f64):
+            lib.set_state(st)
+            lib.upload()
+            lib.forward_device(steps)
+        per = {}
+        for i, f in enumerate(f64.layout.field_names):
+            d, r, m = f32.diff(i, f64)
+            per[f] = {"rel_l2": d / r, "max_abs": m}
+        out[label] = {"steps": steps, "fields": per, "reductions": "FP64",
+                      "fp32_kernel": list(f32.get_kernel()), "fp64_kernel": list(f64.get_kernel())}
+        log(f"{label} FP32 vs FP64: {out[label]}")
+    return out
+
+
+def copy_peak():
+    """Measured HBM copy rate (bytes read + written per second) of a plain
+    16-byte-access copy kernel (formura_amd/csrc/hbm_copy.hip), in GB/s."""
+    import subprocess
+    from formura_amd import build
+    exe = os.path.join(build.BUILD, "hbm_copy")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    for line in r.stdout.splitlines():
+        if line.startswith("copy_peak_GBs"):
+            return float(line.split()[1])
+    return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -259,6 +313,7 @@ def main():
     import numpy as np
     from formura_amd.runtime import load_program
 
+    peak_copy = copy_peak() if not a.variant else None   # before this process's own GPU work
     lib = load_program("heat512")
     lib.init()
     cells = int(np.prod(lib.layout.intra))
@@ -318,7 +373,9 @@ def main():
                                                        "waves": var[3], "prefetch": var[4], "waves_per_simd": var[5],
                                                        "march_chunk": lx, "march_chunk_note": "lx > 0: planes per workgroup run; lx < 0: workgroups per SIMD"}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "measured_copy_peak": peak_copy,
+                     "frac_of_measured_copy_peak": round(achieved / peak_copy, 4) if peak_copy else None},
         "t_sweep_glups": sweep,
         # fraction of the 8 TB/s roofline per fusion depth: a T-fused launch
         # reads and writes the FP64 state once, 16 / T bytes per update
@@ -330,9 +387,17 @@ def main():
     }
 
     if not a.no_extra:
-        extra = {}
-        extra = program_extras()
+        libs = {}
+        extra = program_extras(libs=libs)
         out["extra"] = extra
+        # FP32 bulk against its FP64 twin, same input, the configs' step counts
+        # (heat 512^3: 200 steps; Gray-Scott 512^3: 10000 steps)
+        log("FP32 vs FP64 ...")
+        lib.set_kernel(*var)
+        lib.set_chunk(lx)
+        extra["fp32_vs_fp64"] = precision_errors([
+            ("heat512", libs["heat512_f32"], lib, 200),
+            ("grayscott512", libs["gs3d512_f32"], libs["gs3d512"], 10000)])
         # batched Gray-Scott sweep: 256 uneven 2D instances x 5000 steps, FP32
         # bulk (hipGraph, autotuned plan) vs its FP64 twin and the plain method
         from formura_amd.batched import sweep_benchmark
@@ -348,6 +413,7 @@ def main():
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
     print(json.dumps(out), flush=True)
 
 

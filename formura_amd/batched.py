@@ -21,6 +21,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import build as _build
+from .compiler.emit_batched import FUSED_T
 
 PARAMS = ("F", "k", "dt", "cu", "cv")   # named scalars of examples/gray-scott-2d.fmr
 DU, DV = 2e-5, 1e-5
@@ -92,6 +93,8 @@ class BatchLibrary:
         L.Formura_Batch_Device_Field.argtypes = [ctypes.c_int, ctypes.c_int]
         L.Formura_Batch_Device_Field.restype = ctypes.c_void_p
         L.Formura_Batch_Diff.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, dp]
+        L.Formura_Batch_Fused.argtypes = [ip, ip]
+        L.Formura_Batch_Last_Graph_Nodes.restype = ctypes.c_int
         self.nf = L.Formura_Batch_Num_Fields()
         self.np = L.Formura_Batch_Num_Params()
         self.elem = L.Formura_Batch_Elem_Size()
@@ -148,6 +151,16 @@ class BatchLibrary:
     def advance_plain(self, steps: int) -> float:
         return float(self.lib.Formura_Batch_Advance_Plain(int(steps)))
 
+    def fused(self) -> Optional[Tuple[int, int]]:
+        """(T, lane-width mask) of the fused plan -- every instance advanced
+        by one launch per T-block -- or None (per-bucket launches)."""
+        T, m = ctypes.c_int(), ctypes.c_int()
+        return (T.value, m.value) if self.lib.Formura_Batch_Fused(ctypes.byref(T), ctypes.byref(m)) == 1 else None
+
+    def last_graph_nodes(self) -> int:
+        """Nodes of the hipGraph captured last (advance with graph=True)."""
+        return int(self.lib.Formura_Batch_Last_Graph_Nodes())
+
     def diff(self, i: int, f: int, ref: "BatchLibrary") -> Tuple[float, float, float]:
         r = (ctypes.c_double * 3)()
         ptr = ref.lib.Formura_Batch_Device_Field(i, f)
@@ -172,20 +185,25 @@ def size_buckets(specs: Sequence[InstanceSpec], edges=(256, 512)) -> List[int]:
 
 
 def autotune(lib: BatchLibrary, bucket_of: Sequence[int], probe_steps: int = 24,
-             chunks=(16, 32, 64)) -> Tuple[List[Tuple[int, int, int]], Dict]:
-    """Choose (T, lane width, march chunk) per size bucket by timing every
-    compiled variant on that bucket's instances alone.  Timing advances the
-    state, so callers reload it afterwards.  Returns the plan and the table of
-    rates (GLUP/s)."""
+             chunks=(16, 32, 64), depths=FUSED_T) -> Tuple[List[Tuple[int, int, int]], Dict]:
+    """Choose one fusion depth T for all size buckets and, per bucket, the
+    lane width and march chunk, by timing every compiled (T, lane width) with
+    every chunk on that bucket's instances alone: for each T of `depths`,
+    each bucket takes its fastest (lane width, chunk); the T whose buckets
+    take the least time in all wins.  One depth over all buckets lets one
+    fused kernel advance every instance per T-block; at most two lane widths
+    are used (the fused kernels hold one or two), the two that save the most.
+    Timing advances the state, so callers reload it afterwards.  Returns the
+    plan and the table of rates (GLUP/s)."""
     nb = max(bucket_of) + 1
-    vs = lib.variants()
+    vs = [v for v in lib.variants() if v[0] in depths]
     table: Dict[int, List] = {}
-    plan = [(1, min(vy for _, vy in vs), 64)] * nb
+    plan = [(vs[0][0], vs[0][1], 64)] * nb
+    cells = [sum(s.nx * s.ny for s, bb in zip(lib.specs, bucket_of) if bb == b) for b in range(nb)]
+    rate = {}
     for b in range(nb):
-        cells = sum(s.nx * s.ny for s, bb in zip(lib.specs, bucket_of) if bb == b)
-        if cells == 0:
+        if cells[b] == 0:
             continue
-        best = None
         for T, vy in vs:
             for lx in chunks:
                 trial = list(plan)
@@ -194,11 +212,26 @@ def autotune(lib: BatchLibrary, bucket_of: Sequence[int], probe_steps: int = 24,
                 steps = max(T, (probe_steps // T) * T)
                 lib.advance_bucket(b, steps)
                 ms = lib.advance_bucket(b, steps)
-                rate = steps * cells / (ms * 1e-3) / 1e9
-                table.setdefault(b, []).append(((T, vy, lx), round(rate, 1)))
-                if best is None or rate > best[0]:
-                    best = (rate, (T, vy, lx))
-        plan[b] = best[1]
+                r = steps * cells[b] / (ms * 1e-3) / 1e9
+                rate[(b, T, vy, lx)] = r
+                table.setdefault(b, []).append(((T, vy, lx), round(r, 1)))
+    best = None
+    for T in sorted({t for t, _ in vs}):
+        widths = sorted({vy for t, vy in vs if t == T})
+        for pair in {(a, c) for a in widths for c in widths if a <= c}:
+            cand, total = [], 0.0
+            for b in range(nb):
+                if cells[b] == 0:
+                    cand.append((T, pair[0], 64))
+                    continue
+                opts = [(rate[(b, T, vy, lx)], (T, vy, lx)) for vy in set(pair) for lx in chunks
+                        if (b, T, vy, lx) in rate]
+                r, choice = max(opts)
+                cand.append(choice)
+                total += cells[b] / r           # time per step of this bucket, up to a constant
+            if best is None or total < best[0]:
+                best = (total, cand)
+    plan = best[1]
     lib.plan(bucket_of, plan)
     return plan, table
 
@@ -233,8 +266,17 @@ def sweep_benchmark(steps: int = 5000, plain_steps: int = 200, seed: int = 1000,
         ms = lib.advance(steps, graph=True)
         wall = time.perf_counter() - t0
         libs[name] = lib
+        glups = steps * cells / (ms * 1e-3) / 1e9
+        fz = lib.fused()
+        # roofline: a T-fused launch reads and writes each instance's two
+        # fields once, 2 fields x 2 x element size / T bytes per update
+        T = fz[0] if fz else max(t for t, _, _ in plan)
+        bpu = 2 * 2 * lib.elem / T
         res[name] = {"plan": plan, "gpu_ms": round(ms, 2), "wall_s_incl_capture": round(wall, 3),
-                     "glups": round(steps * cells / (ms * 1e-3) / 1e9, 1)}
+                     "glups": round(glups, 1), "fused": fz, "graph_nodes": lib.last_graph_nodes(),
+                     "launches_per_T_block": 1 if fz else len(plan),
+                     "roofline": {"bound": "hbm", "bytes_per_update": bpu, "achieved_GBs": round(glups * bpu, 1),
+                                  "peak_GBs": 8000.0, "frac": round(glups * bpu / 8000.0, 4)}}
         log(f"{name}: {res[name]}")
     f32, f64 = libs["gs2d_batch_f32"], libs["gs2d_batch_f64"]
     worst, mean = 0.0, 0.0

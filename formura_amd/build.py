@@ -65,6 +65,10 @@ PROGRAMS: Dict[str, tuple] = {
     "fveuler": (_ex("fv-euler-3d.fmr"), _ex("fv-euler-3d.yaml"), "faithful", (), None, None),
     "fveuler_t": (_ex("fv-euler-3d.fmr"), _ex("fv-euler-3d.yaml"), "faithful", (),
                   {"intra_node_shape": [22, 26, 34], "monitor_interval": 2}, None),
+    # an extern function call and index variables (global coordinates) in
+    # the step; and the power operator (pow)
+    "forced_heat": (_ex("forced-heat-3d.fmr"), _ex("forced-heat-3d.yaml"), "faithful", (), None, None),
+    "power_heat": (_ex("power-heat-3d.fmr"), _ex("power-heat-3d.yaml"), "faithful", (), None, None),
 }
 
 
@@ -90,6 +94,7 @@ def _run(cmd: List[str]) -> str:
 
 RESOURCE_FLAG = "-Rpass-analysis=kernel-resource-usage"
 _MARCH_NAME = re.compile(r"march3d_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)E")
+_MARCHV_NAME = re.compile(r"marchv_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELb(\d)ELi(\d+)ELi(\d+)E")
 
 
 def parse_resources(text: str) -> Dict[str, Dict[str, int]]:
@@ -110,12 +115,19 @@ def parse_resources(text: str) -> Dict[str, Dict[str, int]]:
 
 
 def march_resources(res: Dict[str, Dict[str, int]]) -> Dict[tuple, Dict[str, int]]:
-    """(T, rows, waves, prefetch, occupancy) -> resources, for the march kernels."""
+    """(kind, T, rows, waves, prefetch, occupancy) -> resources, for the march
+    kernels (kind 1: march3d_kernel; 3 / 4: marchv_kernel with one / two
+    z-sites per lane)."""
     out = {}
     for name, r in res.items():
         m = _MARCH_NAME.search(name)
         if m:
-            out[tuple(int(x) for x in m.groups())] = r
+            out[(1,) + tuple(int(x) for x in m.groups())] = r
+            continue
+        m = _MARCHV_NAME.search(name)
+        if m:
+            T, by, W, vz, _xc, pf, occ = (int(x) for x in m.groups())
+            out[(2 + vz, T, by, W, pf, occ)] = r
     return out
 
 
@@ -132,7 +144,8 @@ def compile_hip(src: str, out: str, extra: Optional[List[str]] = None) -> str:
 
 
 def resources(name: str) -> Dict[tuple, Dict[str, int]]:
-    """Resources of the march variants compiled into lib<name>.so."""
+    """Resources of the march variants compiled into lib<name>.so, by
+    (kind, T, rows, waves, prefetch, occupancy)."""
     with open(lib_path(name)[:-3] + ".resources.json", encoding="utf-8") as f:
         return {tuple(int(x) for x in k.split(",")): v for k, v in json.load(f).items()}
 
@@ -146,14 +159,16 @@ def build_program(name: str) -> str:
     spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}
     if spilled:
         from formura_amd.compiler.driver import write_hip
-        from formura_amd.compiler.emit_hip import variants_for
-        base = [tuple(v) for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision))]
+        from formura_amd.compiler.emit_hip import _with_kind, variants_for
+        base = [_with_kind(v) for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision))]
         trial = []
         for v in base:
+            kind, T, by, W = v[:4]
+            fewer = (kind, T, by - 1) + v[3:]
             if v not in spilled:
                 trial.append(v)
-            elif v[1] - 1 > 2 * v[0] // v[2] and (v[0], v[1] - 1) + v[2:] not in base:
-                trial.append((v[0], v[1] - 1) + v[2:])
+            elif (by - 1) * W > 2 * T and fewer not in base:
+                trial.append(fewer)
         _, src = write_hip(c, BUILD, trial)
         out = compile_hip(src, lib_path(name))
         spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}

@@ -9,16 +9,26 @@ array per state variable (interior at `[lower, upper)` on every axis), the
 mpi_my_rank and the neighbour ranks), the global `navi`, `Formura_Init` and
 `Formura_Forward`.
 
-State-array margins: the reference sizes them from its region plan
-(MPICxx/Translate.hs:935-969 via Cut.hs); here every axis gets a lower and an
-upper margin equal to the deepest stencil reach of one step. The margins hold
-no interior data; a driver only touches `[lower, upper)`.
+State-array extents and neighbour ranks, as the reference derives them:
+the region plans (normal and wall-inverted, cut_plan.py restating Cut.hs) of
+the T-fold step graph are made, the state arrays get the hull of both plans'
+static allocations with the nanoblock margin `nbu - 1 + 2` added to its upper
+vertex (`collaboratePlans`, MPICxx/Translate.hs:935-969), and the navigator
+declares one `mpi_rank_<delta>` field per rank delta of the plans' ridges and
+its negation, in sorted order (:1032-1035, names by `toCName`, :317-345).
+When no plan can be made -- a config without `initial_walls`, or walls that
+leave regions narrower than the stencil reach, which the reference rejects at
+generation time (Cut.hs:182-214) -- the arrays get a margin of the deepest
+stencil reach of one step on both sides and the navigator every neighbour
+delta.  The margins hold no interior data; a driver only touches
+`[lower, upper)`.  Nanoblock unrolling (`nbu<axis>N`) changes no value; the
+GPU kernels ignore it, and it enters the header only through the margins.
 """
 from __future__ import annotations
 
 import itertools
 from dataclasses import dataclass
-from typing import List
+from typing import List, Optional, Tuple
 
 from .manifest import MMProgram
 from .pointir import PointIR
@@ -30,13 +40,21 @@ class Layout:
     dim: int
     axes: List[str]
     intra: List[int]            # interior extent per axis (one rank)
-    margin: List[int]           # lower margin per axis (= upper margin)
+    margin: List[int]           # lower margin per axis
     field_names: List[str]
     field_ctype: List[str]      # element type of the host arrays
     steps_per_forward: int      # monitorInterval2
+    margin_hi: Optional[List[int]] = None      # upper margin per axis (default: the lower one)
+    deltas: Optional[List[Tuple[int, ...]]] = None   # navigator rank deltas (default: every neighbour)
+
+    def __post_init__(self):
+        if self.margin_hi is None:
+            self.margin_hi = list(self.margin)
+        if self.deltas is None:
+            self.deltas = neighbour_deltas(self.dim)
 
     def shape(self) -> List[int]:
-        return [n + 2 * m for n, m in zip(self.intra, self.margin)]
+        return [n + a + b for n, a, b in zip(self.intra, self.margin, self.margin_hi)]
 
     def lower(self) -> List[int]:
         return list(self.margin)
@@ -56,6 +74,39 @@ def reach_of(mm: MMProgram) -> List[int]:
     return [max(1, v) for v in r]
 
 
+def plan_geometry(mm: MMProgram):
+    """(lower vertex, upper vertex, rank deltas) of the state arrays from the
+    region plans (collaboratePlans, MPICxx/Translate.hs:935-969; deltaMPIs,
+    :1032-1035), or None when no plan can be made for this config."""
+    from .cut_plan import make_plan
+    from .manifest import temporal_blocking
+    nc = mm.numerical_config
+    T = max(1, nc.temporal_blocking_interval)
+    step = temporal_blocking(mm, T) if T > 1 else dict(mm.step)
+    try:
+        plans = [make_plan(step, mm.dimension, list(mm.axes), list(nc.intra_node_shape), nc.initial_walls or {},
+                           mm.state_order, inv, nc.option_strings or []) for inv in (False, True)]
+    except ValueError:
+        return None
+    hull = None
+    for p in plans:
+        for k, b in p.array_alloc.items():
+            if k.static:
+                hull = b if hull is None else hull.hull(b)
+    if hull is None:
+        return None
+    nbu = [nc.nbu_size(a) for a in mm.axes]
+    lo = list(hull.lo)
+    hi = [u + n - 1 + 2 for u, n in zip(hull.hi, nbu)]
+    deltas = set()
+    for p in plans:
+        for r in p.ridge_alloc:
+            d = tuple(r.dmpi)
+            deltas.add(d)
+            deltas.add(tuple(-x for x in d))
+    return lo, hi, sorted(deltas)
+
+
 def make_layout(name: str, mm: MMProgram, ir: PointIR) -> Layout:
     nc = mm.numerical_config
     if nc is None or len(nc.intra_node_shape) != mm.dimension:
@@ -66,8 +117,18 @@ def make_layout(name: str, mm: MMProgram, ir: PointIR) -> Layout:
     for f in mm.state_order:
         e = mm.state_signature[f][2]
         ftypes.append("float" if e == "float" else "double")
-    return Layout(name, mm.dimension, list(mm.axes), list(nc.intra_node_shape), reach_of(mm),
-                  list(mm.state_order), ftypes, nc.monitor_interval2())
+    intra = list(nc.intra_node_shape)
+    reach = reach_of(mm)
+    lay = Layout(name, mm.dimension, list(mm.axes), intra, reach, list(mm.state_order), ftypes,
+                 nc.monitor_interval2())
+    g = plan_geometry(mm)
+    if g is not None:
+        lo, hi, deltas = g
+        m_lo = [-x for x in lo]
+        m_hi = [h - n for h, n in zip(hi, intra)]
+        if all(a >= r and b >= r for a, b, r in zip(m_lo, m_hi, reach)):
+            lay.margin, lay.margin_hi, lay.deltas = m_lo, m_hi, [tuple(d) for d in deltas]
+    return lay
 
 
 def neighbour_deltas(dim: int):
@@ -75,7 +136,11 @@ def neighbour_deltas(dim: int):
 
 
 def delta_name(d) -> str:
-    return "mpi_rank_" + "_".join(("m%d" % -v) if v < 0 else str(v) for v in d)
+    """`nameDeltaMPIRank` (MPICxx/Translate.hs:406-407): `mpi_rank_` and the
+    `toCName` of the rank delta -- its shown form `MPIRank (-1,0)` with `-`
+    written `m`, runs of other characters as one `_`, `MPIRank` dropped and
+    trailing `_` removed, e.g. `mpi_rank__m1_0`."""
+    return "mpi_rank__" + "_".join(("m%d" % -v) if v < 0 else str(v) for v in d)
 
 
 def header_text(lay: Layout) -> str:
@@ -101,7 +166,7 @@ def header_text(lay: Layout) -> str:
     for a in lay.axes:
         out += [f"int lower_{a};", f"int upper_{a};", f"int offset_{a};"]
     out += ["MPI_Comm mpi_comm;", "int mpi_my_rank;"]
-    for d in neighbour_deltas(lay.dim):
+    for d in lay.deltas:
         out.append(f"int {delta_name(d)};")
     out.append("};")
     out.append("extern struct Formura_Navigator navi;")
@@ -116,7 +181,7 @@ def init_body(lay: Layout) -> str:
     (MPICxx/Translate.hs:1070-1092 with mpi_grid_shape all ones: every
     neighbour rank is rank 0, offset = i*intra - o = -o)."""
     lines = ["navi->mpi_comm = comm;", "navi->mpi_my_rank = 0;"]
-    for d in neighbour_deltas(lay.dim):
+    for d in lay.deltas:
         lines.append(f"navi->{delta_name(d)} = 0;")
     lines.append("navi->time_step = 0;")
     for a, o, n in zip(lay.axes, lay.margin, lay.intra):

@@ -1,7 +1,10 @@
-"""ORACLE (test infrastructure only; never part of the product path).
-
-Region plan of the reference's MPI C++ backend, restated from
-src/Formura/MPICxx/Cut.hs:142-616 for the CPU baseline: the grid of each rank
+"""Region plan of the reference's MPI C++ backend, restated from
+src/Formura/MPICxx/Cut.hs:142-616.  The compiler takes the state arrays'
+extents and the navigator's neighbour-rank fields from it (abi.py, as
+MPICxx/Translate.hs:935-969 and 1032-1035 do), and the region-plan CPU
+library of the oracle (oracle/emit_planned.py) its loop and region structure.
+The GPU kernels do not use it: on one GPU the periodic wrap replaces the
+ridges and facets.  The grid of each rank
 is cut by walls (`initial_walls`) into regions ("IRanks"); each manifest node
 of the (temporally blocked) step graph gets its own walls, moved by the
 offsets it reads at, so a region can be computed from the regions before it;

@@ -23,6 +23,27 @@ BATCH_T = (1, 2, 3, 4, 6, 8)
 BATCH_VY = (1, 2, 4)
 
 
+FUSED_T = (1, 2, 4, 8)
+
+
+def fused_kernels(variants: Sequence[Tuple[int, int]]) -> List[Tuple[int, int]]:
+    """(T, lane-width mask) of the fused kernels: for each depth T of
+    FUSED_T, every set of one or two of the lane widths compiled at T (bit 1:
+    one point per lane, 2: two, 4: four), so that a plan of one depth with
+    at most two lane widths over its size buckets runs as one launch per
+    T-block (a remainder of fewer than T steps: 4 + 2 + 1 ...)."""
+    out = []
+    for T in sorted({t for t, _ in variants} & set(FUSED_T)):
+        bits = 0
+        for t, vy in variants:
+            if t == T:
+                bits |= {1: 1, 2: 2, 4: 4}[vy]
+        for mask in range(1, 8):
+            if mask & bits == mask and bin(mask).count("1") <= 2:
+                out.append((T, mask))
+    return out
+
+
 def default_batch_variants(nf: int, real: str) -> List[Tuple[int, int]]:
     words = 2 if real == "double" else 1
     out = []
@@ -68,6 +89,13 @@ def emit_batched(lay: Layout, ir: PointIR, precision: str = "fp32",
         w(f"  {{{{{T}, {vy}}}, &fmr::launch_batched<Prog, {T}, {vy}>}},")
     w("};")
     w(f"constexpr int kNumVariants = {len(variants)};")
+    fused = fused_kernels(variants)
+    w("// fused kernels: depth T, lane widths of the mask (1: one point per lane, 2: two, 4: four)")
+    w("const fmr::FusedEntry<Prog> kFused[] = {")
+    for T, mask in fused:
+        w(f"  {{{T}, {mask}, &fmr::launch_fused<Prog, {T}, {mask}>}},")
+    w("};")
+    w(f"constexpr int kNumFused = {len(fused)};")
     w("fmr::BatchLibrary<Prog> lib;")
     w("bool ready = false;")
     w("}  // namespace")
@@ -82,7 +110,7 @@ def emit_batched(lay: Layout, ir: PointIR, precision: str = "fp32",
     w("{")
     w(f"  if (ready || n < 1 || np != {len(ir.params)}) return -1;")
     w("  for (int i = 0; i < n; ++i) if (nx[i] < 1 || ny[i] < 1) return -1;")
-    w("  lib.init(n, nx, ny, params, np, kVariants, kNumVariants);")
+    w("  lib.init(n, nx, ny, params, np, kVariants, kNumVariants, kFused, kNumFused);")
     w("  ready = true;")
     w("  return 0;")
     w("}")
@@ -144,6 +172,19 @@ def emit_batched(lay: Layout, ir: PointIR, precision: str = "fp32",
     w("{")
     w(f"  if (!ready || i < 0 || i >= lib.n || f < 0 || f >= {nf}) return nullptr;")
     w("  return lib.device_field(i, f);")
+    w("}")
+    w("")
+    w("int Formura_Batch_Fused (int *T, int *mask)")
+    w("{")
+    w("  if (!ready) return -1;")
+    w("  *T = lib.fused_ntiles ? lib.fused_T : 0;")
+    w("  *mask = lib.fused_ntiles ? lib.fused_mask : 0;")
+    w("  return lib.fused_ntiles ? 1 : 0;")
+    w("}")
+    w("")
+    w("int Formura_Batch_Last_Graph_Nodes (void)")
+    w("{")
+    w("  return ready ? lib.last_graph_nodes : -1;")
     w("}")
     w("")
     w("int Formura_Batch_Diff (int i, int f, const void *ref, int ref_elem_size, double *res)")

@@ -131,9 +131,65 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
     return out
 
 
+# kernel kinds (fmr::KernelKind in csrc/fmr_library.hpp)
+KIND_MARCH = 1       # march3d_kernel
+KIND_MARCHV1 = 3     # marchv_kernel, one z-site per lane
+KIND_MARCHV2 = 4     # marchv_kernel, two z-sites per lane
+MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
+MAX_MARCHV_LDS = 144 * 1024   # LDS of one workgroup on gfx950 (160 KB per CU)
+
+
+MARCHV_REG_BUDGET = 250       # of 256 VGPRs (two waves per SIMD)
+
+
+def marchv_regs_per_word(T: int, vz: int) -> float:
+    """VGPRs marchv_kernel takes per z-site of a lane and 32-bit word of the
+    state, as the compiler allocated them for heat and Gray-Scott at T = 3, 4
+    (one site per lane, with the x-carry: 11.3 and 14.9; two sites, three
+    window planes: 10.1 and 12.8); about 24 more for the rest."""
+    return 3.6 * T + 0.5 if vz == 1 else 2.7 * T + 2.0
+
+
+def marchv_lds_bytes(T: int, W: int, nf: int, words: int, vz: int) -> int:
+    """LDS of marchv_kernel's edge exchange: [2][T][W][2][NF][VZ][64] values."""
+    return 2 * T * W * 2 * nf * vz * 64 * 4 * words
+
+
+def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
+    """(kind, T, rows per wave, waves, prefetch depth, waves per SIMD) of
+    marchv_kernel for the fusion depths where it competes (FP64 T = 3, 4;
+    FP32 T = 3..6), at two waves per SIMD (256 VGPRs, no AGPR moves, a second
+    wave to hide latency) in workgroups of 8 waves (a CU): one and two z-sites
+    per lane (two only when nz is even), as many rows per wave as the
+    register model allows (one site per lane: two rows fewer, as slack for
+    the scheduler), and the tile whose stored rows are the largest divisor of
+    ny that fits."""
+    words = 2 if real == "double" else 1
+    depths = range(3, 5) if real == "double" else range(3, 7)
+    out = []
+    for T in depths:
+        for vz, kind in ((1, KIND_MARCHV1), (2, KIND_MARCHV2)):
+            if vz == 2 and nz is not None and nz % 2:
+                continue
+            sites = int((MARCHV_REG_BUDGET - 24) // (nf * words * marchv_regs_per_word(T, vz)))
+            by8 = sites // vz
+            W, by = 8, by8
+            if by >= 1 and W * by - 2 * T >= 4 * T and marchv_lds_bytes(T, W, nf, words, vz) <= MAX_MARCHV_LDS:
+                out.append((kind, T, by, W, 1, 2))
+            if ny and by8 >= 1 and marchv_lds_bytes(T, 8, nf, words, vz) <= MAX_MARCHV_LDS:
+                fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
+                if fits and (kind, T, max(fits), 8, 1, 2) not in out:
+                    out.append((kind, T, max(fits), 8, 1, 2))
+    return out
+
+
 def _march_ny(lay: Layout):
     """The y extent the march kernel tiles (3D programs), else None."""
     return lay.intra[1] if len(lay.intra) == 3 else None
+
+
+def _march_nz(lay: Layout):
+    return lay.intra[2] if len(lay.intra) == 3 else None
 
 
 def device_real(lay: Layout, precision: str) -> str:
@@ -141,8 +197,26 @@ def device_real(lay: Layout, precision: str) -> str:
 
 
 def variants_for(lay: Layout, precision: str):
-    """The march variants emit_hip compiles in by default."""
-    return default_march_variants(len(lay.field_names), device_real(lay, precision), ny=_march_ny(lay))
+    """The march variants emit_hip compiles in by default, as (kind, T, rows,
+    waves, pf, occ): march3d_kernel's -- every T = 1 and T = 2 tile (T = 1:
+    aligned 64-lane rows, only this kernel has them; both: small cache-
+    resident grids such as 64^3, where the latency-hiding single-wave tiles
+    win), and for T >= 3 the tiles at two or more waves per SIMD prefetching
+    one plane (the one-wave-per-SIMD tiles park window values in AGPRs, and a
+    second prefetched plane takes registers; neither won a timed search at
+    512^3), except where only a one-wave-per-SIMD tile fits (FP64 T = 8) --
+    and marchv_kernel's."""
+    nf, real = len(lay.field_names), device_real(lay, precision)
+    base = default_march_variants(nf, real, ny=_march_ny(lay))
+    only_occ1 = {v[0] for v in base} - {v[0] for v in base if v[4] >= 2}
+    old = [(KIND_MARCH,) + tuple(v) for v in base
+           if v[0] <= 2 or (v[4] >= 2 and v[3] == 1) or v[0] in only_occ1]
+    return old + default_marchv_variants(nf, real, ny=_march_ny(lay), nz=_march_nz(lay))
+
+
+def _with_kind(v) -> tuple:
+    v = tuple(v)
+    return v if len(v) == 6 else (KIND_MARCH,) + v
 
 
 def lane_reuse(ir: PointIR, real: str):
@@ -216,7 +290,8 @@ def is_star(ir: PointIR) -> bool:
 
 
 def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
-             march_variants: Sequence[Tuple[int, int]] = None, stages: Sequence[Stage] = ()) -> str:
+             march_variants: Sequence[Tuple[int, int]] = None, stages: Sequence[Stage] = (),
+             extern_header: str = None) -> str:
     """HIP library source of one program: the inlined point function (direct
     and march kernels) and, for a multi-stage program, one stage kernel per
     manifest node (`stages`, from lower_stages)."""
@@ -237,8 +312,13 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     inlined = len(ir.ops) <= INLINE_LIMIT or not has_stages
     use_march = dim == 3 and star and max(r) <= 1 and inlined
     if march_variants is None:
-        march_variants = default_march_variants(len(lay.field_names), real, ny=_march_ny(lay))
-    variants = list(march_variants) if use_march else []
+        march_variants = variants_for(lay, precision)
+    variants = [_with_kind(v) for v in march_variants] if use_march else []
+    if any(v[0] != KIND_MARCH for v in variants):
+        # the x-carry with one z-site per lane; with two, the plane behind is
+        # kept (measured: the carry raised the register peak there)
+        mbody = {1: march_point_body(ir, real, 1), 2: march_point_body(ir, real, 2, use_carry=False)}
+        nc = mbody[1][1]
     n = list(lay.intra) + [1] * (3 - dim)
     shape = lay.shape() + [1] * (3 - dim)
     margin = lay.margin + [0] * (3 - dim)
@@ -251,6 +331,8 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w(f"// precision: {precision}; device element type: {real}")
     w('#include "fmr_library.hpp"')
     w(f'#include "{lay.name}.h"')
+    if extern_header:
+        w(f'#include "{extern_header}"   // the program\'s extern functions (FMR_HD: host and device)')
     w("")
     w("namespace {")
     w("struct Prog {")
@@ -264,6 +346,19 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
         w("  template <class A>")
         w("  __device__ __forceinline__ static void point(A& a) {")
         w("    " + point_body(ir, real))
+        w("  }")
+    if any(v[0] != KIND_MARCH for v in variants):
+        w(f"  // marchv_kernel, one z-site per lane: {nc} value(s) per site carried from the previous plane"
+          + ("" if nc else " (none: the plane behind stays in the window)") + "; two z-sites per lane: none")
+        w(f"  static constexpr int NC = {nc};")
+        w(f"  static constexpr bool XC = {'true' if nc else 'false'};")
+        w("  template <int VZ, class A>")
+        w("  __device__ __forceinline__ static void mpoint(A& a) {")
+        w("    if constexpr (VZ == 1) {")
+        w("      " + mbody[1][0])
+        w("    } else {")
+        w("      " + mbody[2][0])
+        w("    }")
         w("  }")
     else:
         w(f"  // the step inlined into one point function would take {len(ir.ops)} operations per")
@@ -295,9 +390,14 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
         w("};")
         w("")
     w("const fmr::VariantEntry<Prog> kVariants[] = {")
-    for T, by, wv, pf, occ in variants:
-        w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}, {occ}}}, "
-          f"&fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}, {occ}>}},")
+    for kind, T, by, wv, pf, occ in variants:
+        if kind == KIND_MARCH:
+            w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}, {occ}}}, "
+              f"&fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}, {occ}>}},")
+        else:
+            vz = 1 if kind == KIND_MARCHV1 else 2
+            w(f"  {{{{{kind}, {T}, {by}, {wv}, {pf}, {occ}}}, "
+              f"&fmr::launch_marchv<Prog, {T}, {by}, {wv}, {vz}, {pf}, {occ}>}},")
     if not variants:
         w("  {{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}, nullptr},")
     w("};")
@@ -418,15 +518,25 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("{")
     w("  int k = 0;")
     w("  for (int i = 0; i < kNumVariants && k < max_variants; ++i, ++k) {")
-    w("    out[5 * k] = kVariants[i].v.T; out[5 * k + 1] = kVariants[i].v.by; out[5 * k + 2] = kVariants[i].v.waves;")
-    w("    out[5 * k + 3] = kVariants[i].v.pf; out[5 * k + 4] = kVariants[i].v.occ;")
+    w("    out[6 * k] = kVariants[i].v.kind; out[6 * k + 1] = kVariants[i].v.T; out[6 * k + 2] = kVariants[i].v.by;")
+    w("    out[6 * k + 3] = kVariants[i].v.waves; out[6 * k + 4] = kVariants[i].v.pf; out[6 * k + 5] = kVariants[i].v.occ;")
     w("  }")
     w("  return k;")
+    w("}")
+    w("")
+    w("int Formura_Set_Graphs (int on)")
+    w("{")
+    w("  lib.sync();")
+    w("  lib.use_graphs = on != 0;")
+    w("  if (!lib.use_graphs) lib.drop_graphs();")
+    w("  return 0;")
     w("}")
     w("")
     w("int Formura_Set_Params (const double *p, int np)")
     w("{")
     w(f"  if (np < 0 || np > fmr::MAX_PARAMS) return -1;")
+    w("  lib.sync();")
+    w("  lib.drop_graphs();   // captured launches hold the parameters by value")
     w("  for (int i = 0; i < np; ++i) lib.prm.v[i] = p[i];")
     w("  lib.nparams = np;")
     w("  return 0;")
@@ -458,3 +568,103 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("")
     w('}  // extern "C"')
     return "\n".join(out) + "\n"
+
+
+def x_carry(ir: PointIR, real: str):
+    """x-carry plan of the march kernel: ({B: A}, live ops) where op B at
+    site s equals op A at s - e_x (so the march takes B from the value A had
+    at the previous plane), or ({}, live ops) when some load of the plane
+    behind (x offset -1) would stay live -- the window then keeps that plane
+    and nothing is carried.  Lane-axis reuse pairs (lane_reuse) are applied
+    first; the live set follows both substitutions."""
+    sa = ShiftAnalysis(ir)
+    lane, _ = lane_reuse(ir, real)
+    xp = {b: a for b, a in sorted(sa.pairs(0).items())
+          if b not in lane and ir.ops[b].ctype == real and not depends_on(ir, a, b)}
+
+    live = _live_without_carry(ir, lane, xp)
+    carry = {b: a for b, a in xp.items() if b in live}
+    if not carry or any(ir.ops[i].kind == "load" and ir.ops[i].args[1][0] == -1 for i in live):
+        return {}, _live_without_carry(ir, lane)
+    return carry, live
+
+
+def march_point_body(ir: PointIR, real: str, vz: int, use_carry: bool = True) -> Tuple[str, int]:
+    """The per-site statements for `vz` consecutive z-sites of a lane, as the
+    body of `P::mpoint<vz>(a)` of the march kernel of csrc/fmr_march.hpp, and
+    the number of carried values per site.  Op i at site v is `a{i}_{v}`;
+    ops are emitted in op order (each lane-reuse A before its B), all sites of
+    one op together, so that every operand is defined before use:
+      * a lane-reuse B at site v > 0 is A at site v - 1 (same register); at
+        site 0 it is A at the previous lane's last site (`a.lane_prev`);
+      * an x-carried B is the carried value of A from the previous plane,
+        read at the top (`a.xprev<k, v>()`); A's new value is handed back
+        right after it is computed (`a.xkeep<k, v>`)."""
+    lane, order = lane_reuse(ir, real)
+    if use_carry:
+        carry, live = x_carry(ir, real)
+    else:
+        carry, live = {}, _live_without_carry(ir, lane)
+    kidx = {b: k for k, b in enumerate(sorted(carry))}
+    a_of = {carry[b]: kidx[b] for b in carry}
+    lines: List[str] = []
+    names: Dict[Tuple[int, int], str] = {}
+    for b in sorted(carry):
+        for v in range(vz):
+            nm = f"x{kidx[b]}_{v}"
+            lines.append(f"const {ir.ops[b].ctype} {nm} = a.template xprev<{kidx[b]}, {v}>();")
+            names[(b, v)] = nm
+    for i in order:
+        if i not in live:
+            continue
+        op = ir.ops[i]
+        k = op.kind
+        for v in range(vz):
+            nm = f"a{i}_{v}"
+            if i in carry:
+                continue                      # named by its xprev read
+            if k == "load":
+                f, off = op.args
+                o = list(off) + [0] * (3 - len(off))
+                lines.append(f"const {real} {nm} = a.template ld<{f}, {o[0]}, {o[1]}, {o[2]}, {v}>();")
+            elif k == "imm":
+                names[(i, v)] = imm_literal(op.args[0], op.ctype)
+                continue
+            elif k == "param":
+                lines.append(f"const {op.ctype} {nm} = ({op.ctype})a.param({op.args[0]});")
+            elif k == "index":
+                lines.append(f"const {op.ctype} {nm} = ({op.ctype})a.template index<{op.args[0]}, {v}>();")
+            elif i in lane:
+                src = lane[i]
+                if v > 0:
+                    names[(i, v)] = names[(src, v - 1)]
+                    continue
+                lines.append(f"const {op.ctype} {nm} = a.lane_prev({names[(src, vz - 1)]});")
+            else:
+                lines.append(f"const {op.ctype} {nm} = {c_expr_of(op, lambda j: names[(j, v)])};")
+            names[(i, v)] = nm
+            if i in a_of:
+                lines.append(f"a.template xkeep<{a_of[i]}, {v}>({nm});")
+    for v in range(vz):
+        for f, o in enumerate(ir.outputs):
+            lines.append(f"a.template st<{f}, {v}>(({real}){names[(o, v)]});")
+    return "\n      ".join(lines), len(carry)
+
+
+def _live_without_carry(ir: PointIR, lane, carry=None) -> set:
+    """Ops the outputs need when each lane-reuse B (and each carried B) is
+    replaced by its A."""
+    carry = carry or {}
+    live, stack = set(), list(ir.outputs)
+    while stack:
+        i = stack.pop()
+        if i in live:
+            continue
+        live.add(i)
+        if i in carry:
+            stack.append(carry[i])
+        elif i in lane:
+            stack.append(lane[i])
+        else:
+            stack.extend(op_children(ir.ops[i]))
+    return live

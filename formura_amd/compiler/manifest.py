@@ -115,10 +115,11 @@ class _Manifester:
 
 def gen_mm_program(om: OMProgram) -> MMProgram:
     nc = om.numerical_config
-    if nc is not None and any(nc.nbu_size(a) != 1 for a in "xyz"):
-        # nanoblock unroll only replicates the micro DAG at several cursors;
-        # it changes no value, and the GPU kernels choose their own unroll.
-        pass
+    # Nanoblock unroll (`nbu<axis>N`, Manifestation.hs:146-158) replicates
+    # the micro DAG at nbu cursors per loop iteration: it changes no value.
+    # It is honoured where it shows -- the state arrays' upper margin grows
+    # by nbu - 1 (abi.plan_geometry, as MPICxx/Translate.hs:942) -- and the
+    # GPU kernels, which choose their own unrolling, need no more.
     m = _Manifester(om.step_graph, om.dimension)
     step = {k: m.build(k) for k in sorted(m.is_manifest)}
     return MMProgram(om.dimension, om.axes, step, om.state_signature, om.state_order, nc, om)

@@ -31,9 +31,15 @@ struct Instance {
   double prm[MAX_PARAMS];
 };
 
+// One tile of the march: instance, first plane, first stored y, and `meta`
+// = march chunk (planes) * 4 + lane-width selector (0: one point per lane,
+// 1: two, 2: four -- used by the fused kernel; per-bucket launches ignore it).
 struct Tile {
-  int inst, x0, y0, pad;
+  int inst, x0, y0, meta;
 };
+
+__host__ __device__ constexpr int vy_of_sel(int sel) { return sel == 0 ? 1 : (sel == 1 ? 2 : 4); }
+__host__ __device__ constexpr int sel_of_vy(int vy) { return vy == 1 ? 0 : (vy == 2 ? 1 : 2); }
 
 template <class P, int T, int VY>
 struct March2D {
@@ -44,6 +50,7 @@ struct March2D {
   real w[T][3][VY][NF];
   real nxt[VY][NF];
   int lane;
+  int nx, ny;   // the current instance's grid
 
   template <int L, int PH, int V>
   struct Acc : EvalReuse {
@@ -68,7 +75,9 @@ struct March2D {
       } else return m.w[L][CEN][V][F];
     }
     __device__ __forceinline__ double param(int i) const { return prm[i]; }
-    __device__ __forceinline__ int index(int axis) const { return axis == 0 ? cx : cy; }
+    // global coordinates wrapped into the grid: ghost rows and points are
+    // periodic images of interior sites and take their index values
+    __device__ __forceinline__ int index(int axis) const { return axis == 0 ? wrap(cx, m.nx) : wrap(cy, m.ny); }
     template <int F>
     __device__ __forceinline__ void st(real v) { out[F] = v; }
   };
@@ -130,41 +139,72 @@ struct March2D {
   }
 };
 
-// One launch: every tile of `tiles` advanced by T steps, buffers [parity] ->
-// [1 - parity].  Persistent: workgroup b handles tiles b, b + G, b + 2G, ...
+// March one tile by T steps, buffers [parity] -> [1 - parity], over the
+// planes [x0, min(x0 + lx, nx)).
 template <class P, int T, int VY>
-__global__ void __launch_bounds__(64) batched_march2d_kernel(const Instance<typename P::real>* insts,
-                                                             const Tile* tiles, int ntiles, int parity, int lx) {
+__device__ __forceinline__ void march2d_tile(const Instance<typename P::real>& I, const Tile& tl, int parity,
+                                             int lx) {
   using M = March2D<P, T, VY>;
   using real = typename P::real;
   M m;
   m.lane = threadIdx.x;
+  real* const* in = I.buf[parity];
+  real* const* out = I.buf[1 - parity];
+  const int nx = I.nx, ny = I.ny;
+  m.nx = nx;
+  m.ny = ny;
+  const int x0 = tl.x0;
+  const int xend = min(x0 + lx, nx);
+  const int ystart = tl.y0 - T;                 // tile position 0
+  const int ybase = ystart + m.lane * VY;       // unwrapped y of this lane's first point
+  int yoff[VY];
+#pragma unroll
+  for (int v = 0; v < VY; ++v) yoff[v] = wrap(ybase + v, ny);
+  // positions [T, WY - T) are valid; positions past the grid end are
+  // periodic duplicates of the first tile and are not stored
+  const int ylo = T;
+  const int yhi = min(M::WY - T, ny - ystart);
+  const double* prm = I.prm;
+  m.load_row(in, wrap(x0 - T, nx), ny, yoff);
+  const int iters = xend - x0 + 2 * T;
+  for (int s = 0; s < iters; s += 3) {
+    const int xin = x0 - T + s;
+    m.template iter<0>(in, out, prm, xin, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
+    m.template iter<1>(in, out, prm, xin + 1, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
+    m.template iter<2>(in, out, prm, xin + 2, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
+  }
+}
+
+// One launch: every tile of `tiles` (one size bucket) advanced by T steps
+// with VY points per lane and chunks of lx planes.  Persistent: workgroup b
+// handles tiles b, b + G, b + 2G, ...
+template <class P, int T, int VY>
+__global__ void __launch_bounds__(64) batched_march2d_kernel(const Instance<typename P::real>* insts,
+                                                             const Tile* tiles, int ntiles, int parity, int lx) {
   for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
     const Tile tl = tiles[ti];
-    const Instance<real>& I = insts[tl.inst];
-    real* const* in = I.buf[parity];
-    real* const* out = I.buf[1 - parity];
-    const int nx = I.nx, ny = I.ny;
-    const int x0 = tl.x0;
-    const int xend = min(x0 + lx, nx);
-    const int ystart = tl.y0 - T;                 // tile position 0
-    const int ybase = ystart + m.lane * VY;       // unwrapped y of this lane's first point
-    int yoff[VY];
-#pragma unroll
-    for (int v = 0; v < VY; ++v) yoff[v] = wrap(ybase + v, ny);
-    // positions [T, WY - T) are valid; positions past the grid end are
-    // periodic duplicates of the first tile and are not stored
-    const int ylo = T;
-    const int yhi = min(M::WY - T, ny - ystart);
-    const double* prm = I.prm;
-    m.load_row(in, wrap(x0 - T, nx), ny, yoff);
-    const int iters = xend - x0 + 2 * T;
-    for (int s = 0; s < iters; s += 3) {
-      const int xin = x0 - T + s;
-      m.template iter<0>(in, out, prm, xin, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
-      m.template iter<1>(in, out, prm, xin + 1, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
-      m.template iter<2>(in, out, prm, xin + 2, x0, xend, nx, ny, ybase, yoff, ylo, yhi);
-    }
+    march2d_tile<P, T, VY>(insts[tl.inst], tl, parity, lx);
+  }
+}
+
+// The fused launch: every tile of every instance, of all size buckets,
+// advanced by T steps in one persistent kernel.  Each tile carries its own
+// lane width (selector in meta & 3, one of those in the bit mask MASK:
+// 1 = one point per lane, 2 = two, 4 = four) and chunk length (meta >> 2),
+// the per-bucket choices of the plan.  The selector is uniform over the
+// workgroup (one wavefront), so the branch is uniform.
+template <class P, int T, int MASK>
+__global__ void __launch_bounds__(64) batched_fused_kernel(const Instance<typename P::real>* insts,
+                                                           const Tile* tiles, int ntiles, int parity) {
+  for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    const Tile tl = tiles[ti];
+    const int sel = tl.meta & 3, lx = tl.meta >> 2;
+    if constexpr ((MASK & 1) != 0)
+      if (sel == 0) march2d_tile<P, T, 1>(insts[tl.inst], tl, parity, lx);
+    if constexpr ((MASK & 2) != 0)
+      if (sel == 1) march2d_tile<P, T, 2>(insts[tl.inst], tl, parity, lx);
+    if constexpr ((MASK & 4) != 0)
+      if (sel == 2) march2d_tile<P, T, 4>(insts[tl.inst], tl, parity, lx);
   }
 }
 
@@ -213,7 +253,25 @@ struct BatchVariantEntry {
 };
 
 template <class P>
+using FusedLauncher = void (*)(const Instance<typename P::real>*, const Tile*, int, int, int, hipStream_t);
+
+template <class P, int T, int MASK>
+void launch_fused(const Instance<typename P::real>* insts, const Tile* tiles, int ntiles, int parity, int grid,
+                  hipStream_t s) {
+  hipLaunchKernelGGL((batched_fused_kernel<P, T, MASK>), dim3(grid), dim3(WAVE), 0, s, insts, tiles, ntiles, parity);
+}
+
+// a fused kernel: depth T, lane widths of the bit mask `mask`
+template <class P>
+struct FusedEntry {
+  int T, mask;
+  FusedLauncher<P> fn;
+};
+
+template <class P>
 class BatchLibrary {
+  static_assert(P::NF <= MAX_FIELDS, "more state fields than fmr::MAX_FIELDS");
+
  public:
   using real = typename P::real;
   struct Bucket {
@@ -234,20 +292,29 @@ class BatchLibrary {
   std::vector<Bucket> buckets;
   const BatchVariantEntry<P>* variants = nullptr;
   int nvariants = 0;
+  const FusedEntry<P>* fused = nullptr;
+  int nfused = 0;
+  // the fused plan (every bucket of one fusion depth): one tile list over
+  // all instances, one launch per T-block
+  Tile* d_fused = nullptr;
+  int fused_ntiles = 0, fused_T = 0, fused_mask = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int grid = 1024;
   // captured graphs, keyed by step count
   std::vector<std::pair<long, hipGraphExec_t>> graphs;
+  int last_graph_nodes = 0;   // nodes of the graph captured last
   double* red_part = nullptr;
   double* red_res = nullptr;
 
   void init(int count, const int* nx, const int* ny, const double* prm, int nprm, const BatchVariantEntry<P>* vs,
-            int nv) {
+            int nv, const FusedEntry<P>* fs, int nf) {
     n = count;
     np = nprm;
     variants = vs;
     nvariants = nv;
+    fused = fs;
+    nfused = nf;
     FMR_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     FMR_CHECK(hipEventCreate(&ev0));
     FMR_CHECK(hipEventCreate(&ev1));
@@ -278,6 +345,73 @@ class BatchLibrary {
     FMR_CHECK(hipStreamSynchronize(stream));
   }
 
+  // the fused tile list of the current plan, when every bucket has one depth
+  // T and a fused kernel covers the lane widths in use; the instances are
+  // brought to one parity (that of instance 0)
+  void plan_fused() {
+    if (d_fused) FMR_CHECK(hipFree(d_fused));
+    d_fused = nullptr;
+    fused_ntiles = fused_T = fused_mask = 0;
+    int T = 0, mask = 0;
+    for (auto& bk : buckets) {
+      if (bk.insts.empty()) continue;
+      if (T && bk.T != T) return;
+      T = bk.T;
+      mask |= 1 << sel_of_vy(bk.vy);
+    }
+    if (!T || !find_fused(T, mask)) return;
+    const int par = buckets[bucket_of[0]].parity;
+    std::vector<Tile> tl;
+    for (auto& bk : buckets) {
+      for (int i : bk.insts) {
+        if (bk.parity != par)
+          for (int f = 0; f < P::NF; ++f) {
+            const size_t cnt = (size_t)host[i].nx * host[i].ny;
+            FMR_CHECK(hipMemcpyAsync(host[i].buf[par][f], host[i].buf[bk.parity][f], cnt * sizeof(real),
+                                     hipMemcpyDeviceToDevice, stream));
+          }
+        const int wy = WAVE * bk.vy - 2 * bk.T;
+        for (int x0 = 0; x0 < host[i].nx; x0 += bk.lx)
+          for (int y0 = 0; y0 < host[i].ny; y0 += wy) tl.push_back(Tile{i, x0, y0, bk.lx * 4 + sel_of_vy(bk.vy)});
+      }
+      bk.parity = par;
+    }
+    FMR_CHECK(hipMalloc(&d_fused, sizeof(Tile) * tl.size()));
+    FMR_CHECK(hipMemcpyAsync(d_fused, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice, stream));
+    FMR_CHECK(hipStreamSynchronize(stream));
+    fused_ntiles = (int)tl.size();
+    fused_T = T;
+    fused_mask = mask;
+  }
+
+  // one fused launch per T-block, then the remainder at the deepest fused
+  // depth that fits; every bucket's parity flips with each launch
+  void enqueue_fused(int steps, hipStream_t s) {
+    int left = steps;
+    const int g = fused_ntiles < grid ? fused_ntiles : grid;
+    while (left > 0) {
+      const FusedEntry<P>* e = nullptr;
+      for (int t = fused_T <= left ? fused_T : left; t >= 1 && !e; --t) e = find_fused(t, fused_mask);
+      if (!e) {
+        fprintf(stderr, "formura_amd: no fused batched kernel for the remainder\n");
+        abort();
+      }
+      e->fn(d_inst, d_fused, fused_ntiles, buckets[0].parity, g, s);
+      for (auto& bk : buckets) bk.parity ^= 1;
+      left -= e->T;
+    }
+  }
+
+  void advance_parity_fused(int steps) {
+    int left = steps;
+    while (left > 0) {
+      const FusedEntry<P>* e = nullptr;
+      for (int t = fused_T <= left ? fused_T : left; t >= 1 && !e; --t) e = find_fused(t, fused_mask);
+      for (auto& bk : buckets) bk.parity ^= 1;
+      left -= e->T;
+    }
+  }
+
   // release every device resource; the library can be initialised again
   void destroy() {
     FMR_CHECK(hipDeviceSynchronize());
@@ -288,6 +422,9 @@ class BatchLibrary {
       if (bk.join) FMR_CHECK(hipEventDestroy(bk.join));
     }
     buckets.clear();
+    if (d_fused) FMR_CHECK(hipFree(d_fused));
+    d_fused = nullptr;
+    fused_ntiles = fused_T = fused_mask = 0;
     for (auto& I : host)
       for (int b = 0; b < 2; ++b)
         for (int f = 0; f < P::NF; ++f)
@@ -311,6 +448,17 @@ class BatchLibrary {
     for (int i = 0; i < nvariants; ++i)
       if (variants[i].v.T == T && variants[i].v.vy == vy) return &variants[i];
     return nullptr;
+  }
+
+  // the fused kernel of depth T whose lane-width mask covers `mask` with the
+  // fewest extra widths (a width the tiles do not use costs registers)
+  const FusedEntry<P>* find_fused(int T, int mask) const {
+    const FusedEntry<P>* best = nullptr;
+    for (int i = 0; i < nfused; ++i)
+      if (fused[i].T == T && (fused[i].mask & mask) == mask &&
+          (!best || __builtin_popcount(fused[i].mask) < __builtin_popcount(best->mask)))
+        best = &fused[i];
+    return best;
   }
 
   void drop_graphs() {
@@ -369,6 +517,7 @@ class BatchLibrary {
       }
     }
     FMR_CHECK(hipStreamSynchronize(stream));
+    plan_fused();
     return 0;
   }
 
@@ -405,8 +554,13 @@ class BatchLibrary {
     }
   }
 
-  // all buckets on parallel branches: fork from `stream`, join back
+  // the fused launches when the plan has them; otherwise all buckets on
+  // parallel branches: fork from `stream`, join back
   void enqueue_all(int steps) {
+    if (fused_ntiles) {
+      enqueue_fused(steps, stream);
+      return;
+    }
     FMR_CHECK(hipEventRecord(ev0, stream));
     for (auto& bk : buckets) {
       FMR_CHECK(hipStreamWaitEvent(bk.stream, ev0, 0));
@@ -443,9 +597,14 @@ class BatchLibrary {
         FMR_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeGlobal));
         enqueue_all(steps);   // records the launches and advances the parities
         FMR_CHECK(hipStreamEndCapture(stream, &graph));
+        size_t nodes = 0;
+        FMR_CHECK(hipGraphGetNodes(graph, nullptr, &nodes));
+        last_graph_nodes = (int)nodes;
         FMR_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         FMR_CHECK(hipGraphDestroy(graph));
         graphs.push_back({key, exec});
+      } else if (fused_ntiles) {
+        advance_parity_fused(steps);
       } else {
         for (auto& bk : buckets) advance_parity(bk, steps);
       }
@@ -461,8 +620,13 @@ class BatchLibrary {
     return ms;
   }
 
-  // one bucket alone (autotuning): launches on the main stream, no graph
+  // one bucket alone (autotuning): launches on the main stream, no graph;
+  // the bucket's parity then differs from the others', so the fused plan is
+  // rebuilt by the next Formura_Batch_Plan (the autotuner re-plans anyway)
   float advance_bucket(int b, int steps) {
+    if (d_fused) FMR_CHECK(hipFree(d_fused));
+    d_fused = nullptr;
+    fused_ntiles = 0;
     FMR_CHECK(hipStreamSynchronize(stream));
     FMR_CHECK(hipEventRecord(ev0, stream));
     enqueue_bucket(buckets[b], steps, stream);

@@ -4,6 +4,7 @@
 // points declared in include/formura_amd.h.
 #pragma once
 
+#include "fmr_march.hpp"
 #include "fmr_stencil.hpp"
 
 #include <vector>
@@ -72,7 +73,19 @@ __global__ void __launch_bounds__(64) reduce_pass2(const double* part, double* r
 }
 
 // ---- kernel variants --------------------------------------------------------
-enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2 };
+// KIND_MARCH: march3d_kernel (fmr_stencil.hpp); KIND_MARCHV1 / KIND_MARCHV2:
+// marchv_kernel (fmr_march.hpp) with one / two z-sites per lane
+enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2, KIND_MARCHV1 = 3, KIND_MARCHV2 = 4 };
+
+constexpr bool is_march(int kind) { return kind == KIND_MARCH || kind == KIND_MARCHV1 || kind == KIND_MARCHV2; }
+
+// z-sites a wavefront stores per row of a tile: 64 * VZ less the ghost sites
+// (march3d_kernel: none at T = 1, T per side otherwise; marchv_kernel: T per
+// side rounded up to whole lanes)
+constexpr int march_stored_z(int kind, int T) {
+  return kind == KIND_MARCH ? (T == 1 ? WAVE : WAVE - 2 * T)
+                            : (kind == KIND_MARCHV2 ? 2 * WAVE - 2 * ((T + 1) / 2 * 2) : WAVE - 2 * T);
+}
 
 struct Variant {
   int kind;
@@ -96,6 +109,24 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
   }
   hipLaunchKernelGGL((march3d_kernel<P, T, BY, W, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out, nx, ny, nz,
                      nty, ntz, chunk, prm);
+}
+
+template <class P, int T, int BY, int W, int VZ, int PF, int OCC>
+void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
+                   int nty, int ntz, int chunk, Params prm, hipStream_t s) {
+  if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
+    fprintf(stderr, "formura_amd: march kernel needs a plane (NY*NZ sites) smaller than 2 GiB\n");
+    abort();
+  }
+  if (nz % VZ != 0) {
+    fprintf(stderr, "formura_amd: march kernel with %d z-sites per lane needs NZ divisible by %d\n", VZ, VZ);
+    abort();
+  }
+  // the x-carry with one z-site per lane (P::XC); with two, the window keeps
+  // the plane behind (fewer registers at the peak of a row's evaluation)
+  constexpr bool XC = VZ == 1 && P::XC;
+  hipLaunchKernelGGL((marchv_kernel<P, T, BY, W, VZ, XC, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out,
+                     nx, ny, nz, nty, ntz, chunk, prm);
 }
 
 // one manifest node of a multi-stage step (stage_kernel<S>)
@@ -150,6 +181,8 @@ struct VariantEntry {
 
 template <class P>
 class StepLibrary {
+  static_assert(P::NF <= MAX_FIELDS, "more state fields than fmr::MAX_FIELDS");
+
  public:
   using real = typename P::real;
   int n[3] = {1, 1, 1};
@@ -334,7 +367,7 @@ class StepLibrary {
   void launch_march(const VariantEntry<P>* e) {
     const int T = e->v.T, rows = e->v.by * e->v.waves;
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
-    const int zw = T == 1 ? WAVE : WAVE - 2 * T;   // z sites stored per wave (T = 1: no ghost lanes)
+    const int zw = march_stored_z(e->v.kind, T);
     const int ntz = (n[2] + zw - 1) / zw;
     int groups = 0, chunk = 0;
     march_split(nty, ntz, e->v.waves, groups, chunk);
@@ -345,8 +378,8 @@ class StepLibrary {
 
   // the fused variant selected by `choice`, or null (direct kernel, or not compiled in)
   const VariantEntry<P>* main_variant() const {
-    if (choice.kind != KIND_MARCH) return nullptr;
-    return find(KIND_MARCH, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
+    if (!is_march(choice.kind)) return nullptr;
+    return find(choice.kind, choice.T, choice.by, choice.waves, choice.pf, choice.occ);
   }
 
   // launches of the selected kernel for `steps` steps (the rest is done by advance_rest)
@@ -375,7 +408,7 @@ class StepLibrary {
       int depth = 0;
       for (int i = 0; i < nvariants; ++i)
         if (variants[i].v.T <= left && variants[i].v.T > depth) depth = variants[i].v.T;
-      if (depth > 0) e = find(KIND_MARCH, depth, choice.by, choice.waves, choice.pf, choice.occ);
+      if (depth > 0) e = find(choice.kind, depth, choice.by, choice.waves, choice.pf, choice.occ);
       for (int i = 0; i < nvariants && !e; ++i)
         if (variants[i].v.T == depth) e = &variants[i];
       if (!e) {
@@ -393,8 +426,67 @@ class StepLibrary {
     return e ? launches * e->v.T : 0;
   }
 
+  // ---- hipGraphs of the launch sequences --------------------------------
+  // A sequence of launches (the selected kernel's for `steps` steps, or the
+  // remainder) is captured once per (kernel choice, work split, step count,
+  // starting buffer) and replayed: on a grid small enough for the launches to
+  // be short (64^3: a few microseconds each) the host's launch cost would
+  // otherwise dominate.  A replay applies the buffer flips the capture made.
+  struct GraphKey {
+    int kind, T, by, waves, pf, occ, lx, cur, steps, part;
+    bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
+  };
+  struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec;
+    int cur_after, launches;
+  };
+  std::vector<GraphEntry> graphs;
+  bool use_graphs = true;
+
+  void drop_graphs() {
+    for (auto& g : graphs) FMR_CHECK(hipGraphExecDestroy(g.exec));
+    graphs.clear();
+  }
+
+  // run `enqueue` (which returns a launch count and flips `cur`) through a
+  // captured graph; returns the launch count
+  template <class F>
+  int graphed(int steps, int part, F enqueue) {
+    if (!use_graphs || steps <= 0) return enqueue();
+    GraphKey k;
+    memset(&k, 0, sizeof k);
+    k = GraphKey{choice.kind, choice.T, choice.by, choice.waves, choice.pf, choice.occ, lx, cur, steps, part};
+    for (auto& g : graphs)
+      if (g.key == k) {
+        FMR_CHECK(hipGraphLaunch(g.exec, stream));
+        cur = g.cur_after;
+        return g.launches;
+      }
+    hipGraph_t graph;
+    FMR_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const int launches = enqueue();
+    FMR_CHECK(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec;
+    FMR_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    FMR_CHECK(hipGraphDestroy(graph));
+    graphs.push_back(GraphEntry{k, exec, cur, launches});
+    FMR_CHECK(hipGraphLaunch(exec, stream));
+    return launches;
+  }
+
+  int advance_main_g(int steps) {
+    return graphed(steps, 0, [&] { return advance_main(steps); });
+  }
+  void advance_rest_g(int left) {
+    graphed(left, 1, [&] {
+      advance_rest(left);
+      return 0;
+    });
+  }
+
   // advance the device state by `steps` time steps (asynchronous on `stream`)
-  void advance(int steps) { advance_rest(steps - steps_of_main(advance_main(steps))); }
+  void advance(int steps) { advance_rest_g(steps - steps_of_main(advance_main_g(steps))); }
 
   // as advance, timed with events on the library stream: returns the total
   // milliseconds; last_main_ms / last_main_launches time the selected kernel's
@@ -403,9 +495,9 @@ class StepLibrary {
   int last_main_launches = 0;
   float timed_advance(int steps) {
     FMR_CHECK(hipEventRecord(ev0, stream));
-    last_main_launches = advance_main(steps);
+    last_main_launches = advance_main_g(steps);
     FMR_CHECK(hipEventRecord(evm, stream));
-    advance_rest(steps - steps_of_main(last_main_launches));
+    advance_rest_g(steps - steps_of_main(last_main_launches));
     FMR_CHECK(hipEventRecord(ev1, stream));
     FMR_CHECK(hipEventSynchronize(ev1));
     FMR_CHECK(hipEventElapsedTime(&last_main_ms, ev0, evm));

@@ -39,10 +39,17 @@
     }                                                                                \
   } while (0)
 
+// Extern functions of a program are defined in its `<stem>.extern.h` with
+// FMR_HD in front, so that one definition serves the GPU kernels and the
+// host (and, with FMR_HD empty, the CPU oracle).
+#ifndef FMR_HD
+#define FMR_HD __host__ __device__
+#endif
+
 namespace fmr {
 
 constexpr int WAVE = 64;
-constexpr int MAX_FIELDS = 8;
+constexpr int MAX_FIELDS = 16;
 constexpr int MAX_PARAMS = 16;
 constexpr int N_XCD = 8;   // MI355X: 8 accelerator complex dies, each with its own L2
 
@@ -340,6 +347,7 @@ struct March3D {
   real hnxt[PF][NH][NF];  // and of the prefetched planes
   Edge* edge;   // LDS (W > 1 only)
   int wave, lane, half;
+  int nx, ny;          // grid extents (index values wrap into them)
   unsigned zoff;       // byte offset of this lane's z-column within a row
   unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane (the store is dropped)
   unsigned roff[BY];   // byte offset of row r within a plane (uniform: scalar registers)
@@ -397,7 +405,12 @@ struct March3D {
       }
     }
     __device__ __forceinline__ double param(int i) const { return prm.v[i]; }
-    __device__ __forceinline__ int index(int axis) const { return axis == 0 ? cx : (axis == 1 ? cy : cz); }
+    // global coordinates of the site, wrapped into [0, n): ghost planes, rows
+    // and lanes and the rows past the grid end of the last tile are periodic
+    // images of interior sites and take those sites' index values
+    __device__ __forceinline__ int index(int axis) const {
+      return axis == 0 ? wrap(cx, m.nx) : (axis == 1 ? wrap(cy, m.ny) : cz);
+    }
     template <int F>
     __device__ __forceinline__ void st(real v) { out[F] = v; }
     // the lane axis is z: the previous site's value is in the previous lane;
@@ -611,6 +624,8 @@ march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx
   m.wave = threadIdx.x / WAVE;
   m.lane = threadIdx.x % WAVE;
   m.half = 0;
+  m.nx = nx;
+  m.ny = ny;
   if constexpr (W > 1) {
     __shared__ typename M::Edge edge_buf;
     m.edge = &edge_buf;

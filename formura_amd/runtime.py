@@ -17,15 +17,17 @@ from typing import Dict, List, Sequence
 
 import numpy as np
 
-from .compiler.abi import Layout, delta_name, neighbour_deltas
+from .compiler.abi import Layout, delta_name
 
 
-def navigator_type(axes: Sequence[str]):
+def navigator_type(layout: Layout):
+    """struct Formura_Navigator of a library with this layout (field order
+    of the generated header)."""
     fields = [("time_step", ctypes.c_int)]
-    for a in axes:
+    for a in layout.axes:
         fields += [(f"lower_{a}", ctypes.c_int), (f"upper_{a}", ctypes.c_int), (f"offset_{a}", ctypes.c_int)]
     fields += [("mpi_comm", ctypes.c_int), ("mpi_my_rank", ctypes.c_int)]
-    for d in neighbour_deltas(len(axes)):
+    for d in layout.deltas:
         fields.append((delta_name(d), ctypes.c_int))
     return type("Formura_Navigator", (ctypes.Structure,), {"_fields_": fields})
 
@@ -40,7 +42,7 @@ class StepLibrary:
         self.path = path
         self.layout = layout
         self.lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
-        self.Nav = navigator_type(layout.axes)
+        self.Nav = navigator_type(layout)
         self.navi = self.Nav.in_dll(self.lib, "navi")
         shape = layout.shape()
         self.arrays: Dict[str, np.ndarray] = {}
@@ -68,6 +70,7 @@ class StepLibrary:
             L.Formura_Num_Stages.restype = ctypes.c_int
             L.Formura_Num_Stage_Kernels.restype = ctypes.c_int
             L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+            L.Formura_Set_Graphs.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.restype = ctypes.c_void_p
             L.Formura_Reduce_Device.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -145,10 +148,16 @@ class StepLibrary:
         return tuple(x.value for x in v)
 
     def list_kernels(self) -> List[tuple]:
-        """Compiled march variants as (T, rows per wave, waves, prefetch depth, waves per SIMD)."""
-        buf = (ctypes.c_int * 320)()
-        n = self.lib.Formura_List_Kernels(buf, 64)
-        return [tuple(buf[5 * i + j] for j in range(5)) for i in range(n)]
+        """Compiled march variants as (kind, T, rows per wave, waves, prefetch
+        depth, waves per SIMD), ready for set_kernel(*v)."""
+        buf = (ctypes.c_int * (6 * 256))()
+        n = self.lib.Formura_List_Kernels(buf, 256)
+        return [tuple(buf[6 * i + j] for j in range(6)) for i in range(n)]
+
+    def set_graphs(self, on: bool) -> None:
+        """Capture and replay the launch sequences as hipGraphs (default), or
+        launch eagerly."""
+        self.lib.Formura_Set_Graphs(1 if on else 0)
 
     def set_params(self, p: Sequence[float]) -> None:
         arr = (ctypes.c_double * len(p))(*p)
@@ -182,8 +191,11 @@ class StepLibrary:
 
 
 KIND_DIRECT = 0
-KIND_MARCH = 1
+KIND_MARCH = 1       # temporally blocked march, one z-site per lane (march3d_kernel)
 KIND_STAGED = 2      # multi-stage programs: one kernel per manifest node
+KIND_MARCHV1 = 3     # march with the x-carry, one z-site per lane (marchv_kernel)
+KIND_MARCHV2 = 4     # march with the x-carry, two z-sites per lane (marchv_kernel)
+MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
 
 
 def march_chunks(occ: int) -> tuple:

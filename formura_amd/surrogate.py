@@ -197,33 +197,42 @@ def trajectory_specs(n: int, seed: int) -> List[InstanceSpec]:
 
 class Stencil:
     """The FP32 stencil for the surrogate's instances: the batched library
-    (gs2d_batch_f32) holding one instance per trajectory."""
+    (gs2d_batch_f32) holding one instance per trajectory and, with
+    `shadows`, that many more: the shadow instances of the checked rollout,
+    in a second size bucket of the same plan."""
 
-    def __init__(self, specs: Sequence[InstanceSpec], plan=(2, 2, 32), name: str = "gs2d_batch_f32"):
+    def __init__(self, specs: Sequence[InstanceSpec], plan=(2, 2, 32), name: str = "gs2d_batch_f32",
+                 shadows: Sequence[int] = ()):
         self.lib = open_batch(name)
-        self.lib.init(specs)
         self.specs = list(specs)
-        self.lib.plan([0] * len(specs), [plan])
+        self.shadows = list(shadows)
+        self.lib.init(self.specs + [self.specs[i] for i in self.shadows])
+        self.lib.plan([0] * len(self.specs) + [1] * len(self.shadows), [plan, plan] if self.shadows else [plan])
+
+    def _ptrs(self, first: int, count: int):
+        ptrs = []
+        for j in range(count):
+            for f in range(2):
+                ptrs.append(self.lib.lib.Formura_Batch_Device_Field(first + j, f))
+        return ptrs
 
     def load(self, sur: Surrogate, src: DeviceBuffer, first: int = 0, count: Optional[int] = None) -> None:
         """Copy states [count][2][H][W] from a device buffer into instances first..first+count-1."""
         count = len(self.specs) - first if count is None else count
-        ptrs = []
-        for j in range(count):
-            for f in range(2):
-                ptrs.append(self.lib.lib.Formura_Batch_Device_Field(first + j, f))
-        sur.scatter(ptrs, src, H * W)
+        sur.scatter(self._ptrs(first, count), src, H * W)
 
     def store(self, sur: Surrogate, dst: DeviceBuffer, first: int = 0, count: Optional[int] = None) -> None:
         count = len(self.specs) - first if count is None else count
-        ptrs = []
-        for j in range(count):
-            for f in range(2):
-                ptrs.append(self.lib.lib.Formura_Batch_Device_Field(first + j, f))
-        sur.gather(ptrs, dst, H * W)
+        sur.gather(self._ptrs(first, count), dst, H * W)
 
     def advance(self, steps: int) -> float:
+        """Advance the trajectory instances (and the shadows with them)."""
+        if self.shadows:
+            return self.lib.advance_bucket(0, steps)
         return self.lib.advance(steps, graph=True)
+
+    def advance_shadows(self, steps: int) -> float:
+        return self.lib.advance_bucket(1, steps)
 
 
 def fk_array(specs: Sequence[InstanceSpec]) -> np.ndarray:
@@ -422,6 +431,7 @@ class Rollout:
     stencil_supersteps: int = 0
     checks: List[float] = field(default_factory=list)
     fallback_at: Optional[int] = None
+    accumulated: List[float] = field(default_factory=list)   # shadow checks, one per window
     seconds: float = 0.0
     windows: List[np.ndarray] = field(default_factory=list)   # state after each window (keep_windows)
 
@@ -430,18 +440,33 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
                     check_every: int = CHECK_EVERY, bound: float = BOUND, keep_windows: bool = False) -> Rollout:
     """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each).
 
-    Windows of `check_every` super-steps: the CNN advances the window; its last
-    super-step is also computed by the FP32 stencil from the same state (the
-    check). If the relative L2 difference is within `bound` for every
-    trajectory, the stencil's result closes the window. Otherwise the window
-    is recomputed from its start state by the stencil, which then advances
-    the rest of the rollout."""
+    Windows of `check_every` super-steps: the CNN advances the window.  Two
+    checks close a window:
+      * local: its last super-step is also computed by the FP32 stencil from
+        the same state; the relative L2 difference must be within `bound` for
+        every trajectory, and the stencil's result closes the window;
+      * accumulated: the stencil's own trajectories of the shadow instances
+        (st.shadows, a sample of the trajectories, advanced by the stencil
+        alone from the rollout's start) against the rollout's state of the
+        same trajectories after the window; the relative L2 difference must
+        be within `bound` -- this sees the error the unchecked super-steps
+        accumulated, which the local check cannot.
+    When a check fails, the window is recomputed from its start state by the
+    stencil, which then advances the rest of the rollout."""
     B = x0.shape[0]
     nbytes = x0.nbytes
+    nb1 = nbytes // B
+    k = len(st.shadows)
     a, b, ref, ckpt = sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes), sur.buffer(nbytes)
+    samp = sur.buffer(nb1 * k) if k else None
+    shad = sur.buffer(nb1 * k) if k else None
     a.upload(x0)
     res = Rollout(state=None)
     t0 = time.perf_counter()
+    if k:                                          # the shadows start where their trajectories do
+        for q, i in enumerate(st.shadows):
+            sur.lib.Sur_Copy(ctypes.c_void_p(shad.ptr + q * nb1), ctypes.c_void_p(a.ptr + i * nb1), nb1, 2)
+        st.load(sur, shad, B, k)
     j = 0
     while j < n_super:
         w = min(check_every, n_super - j)
@@ -467,7 +492,15 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
         res.stencil_supersteps += 1
         err = max(sur.rel_l2(b, ref, B, x0[0].size))
         res.checks.append(err)
-        if err <= bound:
+        acc = 0.0
+        if k:
+            st.advance_shadows(SUPER * w)
+            st.store(sur, shad, B, k)
+            for q, i in enumerate(st.shadows):
+                sur.lib.Sur_Copy(ctypes.c_void_p(samp.ptr + q * nb1), ctypes.c_void_p(ref.ptr + i * nb1), nb1, 2)
+            acc = max(sur.rel_l2(samp, shad, k, x0[0].size))
+            res.accumulated.append(acc)
+        if err <= bound and acc <= bound:
             a, ref = ref, a
         else:                                     # roll back and redo the window with the stencil
             res.fallback_at = j
@@ -483,9 +516,18 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
     # copy to the host is not timed, as for the plain stencils it is set against
     res.seconds = time.perf_counter() - t0
     res.state = a.download(x0.shape)
-    for buf in (a, b, ref, ckpt):
-        buf.free()
+    for buf in (a, b, ref, ckpt, samp, shad):
+        if buf is not None:
+            buf.free()
     return res
+
+
+SHADOWS = 2     # shadow instances of a checked rollout (the accumulated-error check)
+
+
+def shadow_sample(n_traj: int, k: int = SHADOWS) -> List[int]:
+    """The trajectories the shadow instances follow: k spread evenly."""
+    return sorted({(q * n_traj) // k for q in range(min(k, n_traj))})
 
 
 def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed: int = 99,
@@ -493,9 +535,12 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     """Checked surrogate vs the stencil on held-out trajectories.
 
     Reference: the FP64 batch library (bit-equal to the CPU oracle) run for
-    the same 32*n_super steps.  Timings: the checked surrogate; the plain FP32
-    and FP64 stencils (one single-site-per-thread launch per instance per
-    step); the batched FP32 stencil (fused, hipGraph)."""
+    the same 32*n_super steps.  Timings, all from the same start state and
+    ending with the state in device memory: the checked surrogate (local and
+    accumulated checks, shadow_sample); the batched FP32 and FP64 stencils
+    (fused, hipGraph) -- the comparators; and the plain FP32 and FP64
+    stencils (one single-site-per-thread launch per instance per step), a
+    launch-bound reference."""
     sur = Surrogate.get(max(16, n_traj))
     sur.set_params(theta)
     sur.set_scale(*scale)
@@ -512,7 +557,8 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     start = sur.buffer(x0.nbytes)
     st.store(sur, start)
     xs = start.download(x0.shape)
-    out: Dict = {"trajectories": n_traj, "super_steps": n_super, "steps": steps, "bound": bound}
+    out: Dict = {"trajectories": n_traj, "super_steps": n_super, "steps": steps, "bound": bound,
+                 "check_every": CHECK_EVERY, "shadow_instances": shadow_sample(n_traj)}
     # FP64 reference
     ref64 = open_batch("gs2d_batch_f64")
     ref64.init(specs)
@@ -537,27 +583,42 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     out["one_step_rel_l2"] = {"max": max(errs), "mean": float(np.mean(errs))}
     log(f"one super-step, network vs stencil: {out['one_step_rel_l2']}")
     # checked surrogate (warm-up first: one forward and one stencil super-step)
+    st = Stencil(specs, shadows=shadow_sample(n_traj))
     st.load(sur, start)
     st.advance(SUPER)
     ro = checked_rollout(sur, st, fk, xs, n_super, bound=bound)
     out["surrogate"] = {"seconds": round(ro.seconds, 4), "cnn_supersteps": ro.cnn_steps,
                         "stencil_supersteps": ro.stencil_supersteps, "fallback_at": ro.fallback_at,
                         "check_errors": [float(f"{e:.3e}") for e in ro.checks],
+                        "accumulated_errors": [float(f"{e:.3e}") for e in ro.accumulated],
                         "rel_l2_vs_fp64": rel_l2(ro.state.astype(np.float64), truth)}
     # the same rollout again (untimed), keeping the state after each window,
     # for the error against the FP64 run as the rollout goes on
-    st.load(sur, start)
     ro2 = checked_rollout(sur, st, fk, xs, n_super, bound=bound, keep_windows=True)
     out["surrogate"]["rel_l2_vs_fp64_by_window"] = [
         float(f"{rel_l2(w.astype(np.float64), t):.3e}") for w, t in zip(ro2.windows, truths)]
     out["surrogate"]["steps_by_window"] = [SUPER * min(n_super, j + CHECK_EVERY) for j in range(0, n_super, CHECK_EVERY)]
     log(f"checked surrogate: {out['surrogate']}")
-    # plain stencils and the batched stencil from the same state
+
+    def time_batched(name, nsteps):
+        """The batched stencil (fused plan, hipGraph) from the start state."""
+        lib = open_batch(name)
+        lib.init(specs)
+        lib.plan([0] * n_traj, [(8, 2, 32)])
+        for i in range(n_traj):
+            lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+        lib.advance(nsteps)   # capture
+        for i in range(n_traj):
+            lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
+        t0 = time.perf_counter()
+        lib.advance(nsteps)
+        sec = time.perf_counter() - t0
+        return sec, np.stack([np.stack(lib.get_state(i)) for i in range(n_traj)]), lib.fused()
+
     def time_plain(name, nsteps):
         lib = open_batch(name)
-        if lib.specs != specs:
-            lib.init(specs)
-            lib.plan([0] * n_traj, [(2, 2, 32)])
+        lib.init(specs)
+        lib.plan([0] * n_traj, [(2, 2, 32)])
         for i in range(n_traj):
             lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
         lib.advance_plain(2)
@@ -568,41 +629,29 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
         sec = time.perf_counter() - t0
         return sec, np.stack([np.stack(lib.get_state(i)) for i in range(n_traj)])
 
+    for name, key in (("gs2d_batch_f32", "batched_fp32"), ("gs2d_batch_f64", "batched_fp64")):
+        sec, res, fz = time_batched(name, steps)
+        out[key] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth), "fused": fz}
+        log(f"{key}: {out[key]}")
     for name, key in (("gs2d_batch_f32", "plain_fp32"), ("gs2d_batch_f64", "plain_fp64")):
         sec, res = time_plain(name, steps)
         out[key] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth)}
         log(f"{key}: {out[key]}")
+    s_ = out["surrogate"]["seconds"]
+    out["speedup_vs_batched_fp32"] = round(out["batched_fp32"]["seconds"] / s_, 3)
+    out["speedup_vs_batched_fp64"] = round(out["batched_fp64"]["seconds"] / s_, 3)
+    out["speedup_vs_plain_fp32_launch_bound"] = round(out["plain_fp32"]["seconds"] / s_, 2)
+    out["speedup_vs_plain_fp64_launch_bound"] = round(out["plain_fp64"]["seconds"] / s_, 2)
     # the horizon over which the checked surrogate stays within the bound of
-    # the FP64 run, timed on its own against the plain FP64 stencil
+    # the FP64 run: the longest prefix of windows whose error is within it
     sw = out["surrogate"]
-    ok = [(n, e) for n, e in zip(sw["steps_by_window"], sw["rel_l2_vs_fp64_by_window"]) if e <= bound]
-    if ok:
-        h, e_h = max(ok)
-        st.load(sur, start)
-        ro3 = checked_rollout(sur, st, fk, xs, h // SUPER, bound=bound)
-        sec64, _ = time_plain("gs2d_batch_f64", h)
-        sec32, _ = time_plain("gs2d_batch_f32", h)
-        out["within_bound"] = {"steps": h, "rel_l2_vs_fp64": e_h, "surrogate_seconds": round(ro3.seconds, 4),
-                               "plain_fp64_seconds": round(sec64, 4), "plain_fp32_seconds": round(sec32, 4),
-                               "speedup_vs_plain_fp64": round(sec64 / ro3.seconds, 2),
-                               "speedup_vs_plain_fp32": round(sec32 / ro3.seconds, 2)}
-        log(f"within the bound: {out['within_bound']}")
-    st.lib.plan([0] * n_traj, [(8, 2, 32)])
-    for i in range(n_traj):
-        st.lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
-    st.advance(steps)   # capture
-    for i in range(n_traj):
-        st.lib.set_state(i, [xs[i, 0].astype(np.float64), xs[i, 1].astype(np.float64)])
-    t0 = time.perf_counter()
-    st.advance(steps)
-    sec = time.perf_counter() - t0
-    res = np.stack([np.stack(st.lib.get_state(i)) for i in range(n_traj)])
-    out["batched_fp32"] = {"seconds": round(sec, 4), "rel_l2_vs_fp64": rel_l2(res, truth)}
-    log(f"batched_fp32: {out['batched_fp32']}")
-    s = out["surrogate"]["seconds"]
-    out["speedup_vs_plain_fp32"] = round(out["plain_fp32"]["seconds"] / s, 2)
-    out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["seconds"] / s, 2)
-    out["speedup_vs_batched_fp32"] = round(out["batched_fp32"]["seconds"] / s, 3)
+    h = 0
+    for n, e in zip(sw["steps_by_window"], sw["rel_l2_vs_fp64_by_window"]):
+        if e > bound:
+            break
+        h = n
+    out["within_bound_steps"] = h
+    out["within_bound_full_rollout"] = h == steps
     for b in (fk, start):
         b.free()
     return out

@@ -83,6 +83,12 @@ int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *o
  * march variants into out[5*i .. 5*i+4]; returns the number written. */
 int Formura_List_Kernels(int *out, int max_variants);
 
+/* on != 0 (the default): the launch sequence of a Formura_Forward_Device*
+ * call is captured in a hipGraph once per (kernel, work split, step count,
+ * starting buffer) and replayed; 0: launched eagerly (and the graphs are
+ * released).  Returns 0. */
+int Formura_Set_Graphs(int on);
+
 /* Per-launch scalar parameters (programs compiled with named parameters);
  * returns 0, or -1 if np exceeds 16. */
 int Formura_Set_Params(const double *p, int np);

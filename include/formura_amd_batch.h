@@ -37,7 +37,9 @@ int Formura_Batch_List_Variants(int *out, int max_variants);
 
 /* Group the instances into nbuckets buckets (bucket_of_instance[i]) and give
  * bucket b the variant (T, points per lane, march chunk) =
- * variant[3*b .. 3*b+2].  Keeps the instances' current state. */
+ * variant[3*b .. 3*b+2].  Keeps the instances' current state.  When every
+ * bucket has the same T (and at most two lane widths are used), the plan is
+ * fused. */
 int Formura_Batch_Plan(int nbuckets, const int *bucket_of_instance, const int *variant);
 
 /* Copy field f of instance i from / to a host array of nx*ny doubles. */
@@ -46,8 +48,22 @@ int Formura_Batch_Get_State(int i, int f, double *dst);
 
 /* Advance every instance by nsteps; use_graph != 0 captures the launch
  * sequence in a hipGraph on first use (per step count and starting parity)
- * and replays it.  Returns GPU milliseconds (HIP events around the work). */
+ * and replays it.  With a fused plan (every bucket of one depth T, see
+ * Formura_Batch_Fused) each T-block is one persistent launch over the tiles
+ * of all instances; otherwise the buckets run on parallel branches.  Returns
+ * GPU milliseconds (HIP events around the work). */
 float Formura_Batch_Advance(int nsteps, int use_graph);
+
+/* 1 if the current plan is fused -- one kernel launch advances every
+ * instance of every bucket by T steps, each tile with its bucket's lane
+ * width and march chunk -- with *T its depth and *mask its lane widths
+ * (bit 1: one point per lane, 2: two, 4: four); 0 if the buckets' depths
+ * differ (per-bucket launches), -1 before Formura_Batch_Init. */
+int Formura_Batch_Fused(int *T, int *mask);
+
+/* Number of nodes of the hipGraph captured last by Formura_Batch_Advance
+ * (a fused plan: one kernel node per T-block). */
+int Formura_Batch_Last_Graph_Nodes(void);
 
 /* Advance only the instances of bucket b (for autotuning); GPU ms. */
 float Formura_Batch_Advance_Bucket(int b, int nsteps);

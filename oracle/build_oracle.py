@@ -22,7 +22,7 @@ ROOT = os.path.dirname(HERE)
 sys.path.insert(0, ROOT)
 
 from formura_amd import build as product_build  # noqa: E402
-from formura_amd.compiler.driver import compile_files  # noqa: E402
+from formura_amd.compiler.driver import compile_files, copy_extern_header  # noqa: E402
 from oracle.emit_cxx import write_oracle  # noqa: E402
 from oracle.emit_planned import write_planned  # noqa: E402
 
@@ -36,8 +36,9 @@ _GS2D_PARAMS = ("F", "k", "dt", "cu", "cv")
 ORACLE_ONLY = {
     # the 512^3 heat program with 10 steps per Formura_Forward: the CPU-baseline sample
     "heat512_cpu": ("heat512", {"monitor_interval": 10}, ()),
-    # Gray-Scott 512^3 with 10 steps per Formura_Forward: full-size GPU parity
-    "gs3d512_cpu": ("gs3d512", {"monitor_interval": 10}, ()),
+    # Gray-Scott 512^3 with 12 steps per Formura_Forward (10 rounded up to a
+    # multiple of 2T, T = 2): full-size GPU parity
+    "gs3d512_cpu": ("gs3d512", {"monitor_interval": 12}, ()),
     # 2D Gray-Scott with per-run parameters, for the batched-launch parity tests
     "gs2d_p256": ("gs2d", {"monitor_interval": 40}, _GS2D_PARAMS),
     "gs2d_p96x160": ("gs2d", {"intra_node_shape": [96, 160], "monitor_interval": 40}, _GS2D_PARAMS),
@@ -121,7 +122,7 @@ def compiled_extra(name: str):
 def build_oracle(name: str) -> str:
     c = compiled_extra(name) if name in ORACLE_ONLY else compiled(name)
     params = ORACLE_ONLY[name][2] if name in ORACLE_ONLY else ()
-    src = write_oracle(OUT, c.layout, c.mm, params)
+    src = write_oracle(OUT, c.layout, c.mm, params, copy_extern_header(c, OUT))
     out = oracle_path(name)
     cmd = ["gcc"] + CFLAGS + ["-I", OUT, "-I", product_build.MPI_STUB, src, "-o", out + ".tmp", "-lm"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -54,7 +54,7 @@ def _param_nodes(mm: MMProgram, params) -> Dict[int, int]:
     return out
 
 
-def emit_cxx(lay: Layout, mm: MMProgram, params=()) -> str:
+def emit_cxx(lay: Layout, mm: MMProgram, params=(), extern_header=None) -> str:
     dim = lay.dim
     axes = lay.axes
     shape = lay.shape()
@@ -68,6 +68,8 @@ def emit_cxx(lay: Layout, mm: MMProgram, params=()) -> str:
     w("#include <stdbool.h>")
     w("#include <string.h>")
     w(f'#include "{lay.name}.h"')
+    if extern_header:
+        w(f'#include "{extern_header}"')
     w("")
     for f, t in zip(lay.field_names, lay.field_ctype):
         w(f"{t} {f}{dims_decl};")
@@ -205,11 +207,11 @@ def emit_cxx(lay: Layout, mm: MMProgram, params=()) -> str:
     return "\n".join(out) + "\n"
 
 
-def write_oracle(outdir: str, lay: Layout, mm: MMProgram, params=()) -> str:
+def write_oracle(outdir: str, lay: Layout, mm: MMProgram, params=(), extern_header=None) -> str:
     os.makedirs(outdir, exist_ok=True)
     with open(os.path.join(outdir, f"{lay.name}.h"), "w") as f:
         f.write(header_text(lay))
     path = os.path.join(outdir, f"{lay.name}_cpu.c")
     with open(path, "w") as f:
-        f.write(emit_cxx(lay, mm, params))
+        f.write(emit_cxx(lay, mm, params, extern_header))
     return path

@@ -1,7 +1,7 @@
 """ORACLE (test infrastructure only; never part of the product path).
 
 C/OpenMP step library on the reference's own loop and region structure:
-the region plans of oracle/cut_plan.py (normal and wall-inverted, Cut.hs)
+the region plans of formura_amd/compiler/cut_plan.py (normal and wall-inverted, Cut.hs)
 turned into C as the reference's MPI C++ backend does
 (src/Formura/MPICxx/Translate.hs):
 
@@ -38,28 +38,18 @@ from typing import Dict, List
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."))
 
-from formura_amd.compiler.abi import Layout  # noqa: E402
+from formura_amd.compiler.abi import Layout, delta_name, header_text  # noqa: E402
 from formura_amd.compiler.manifest import MMProgram, temporal_blocking  # noqa: E402
 from formura_amd.compiler.pointir import C_TYPES, Op, c_expr_of, imm_literal  # noqa: E402
-from oracle.cut_plan import Box, FacetID, IRank, Plan, RidgeID, Rsc, make_plan  # noqa: E402
+from formura_amd.compiler.cut_plan import Box, FacetID, IRank, Plan, RidgeID, Rsc, make_plan  # noqa: E402
 
 
-class PlannedLayout(Layout):
-    """Layout of a library with the reference's (asymmetric) array margins."""
-
-    def __init__(self, base: Layout, lower, shape):
-        super().__init__(base.name, base.dim, base.axes, base.intra, list(lower), base.field_names,
-                         base.field_ctype, base.steps_per_forward)
-        self._shape = list(shape)
-
-    def shape(self):
-        return list(self._shape)
-
-    def lower(self):
-        return list(self.margin)
-
-    def upper(self):
-        return [m + n for m, n in zip(self.margin, self.intra)]
+def PlannedLayout(base: Layout, lower, shape, deltas) -> Layout:
+    """Layout of a library with the plan's (asymmetric) array margins and
+    rank deltas."""
+    hi = [sh - n - lo for sh, n, lo in zip(shape, base.intra, lower)]
+    return Layout(base.name, base.dim, base.axes, base.intra, list(lower), base.field_names, base.field_ctype,
+                  base.steps_per_forward, hi, [tuple(d) for d in deltas])
 
 
 def _ctype(elem: str) -> str:
@@ -90,7 +80,7 @@ def facet_name(f: FacetID) -> str:
 
 
 def delta_field(d) -> str:
-    return "mpi_rank_" + _cname(_vec(d))
+    return delta_name(d)
 
 
 class _Emitter:
@@ -298,29 +288,11 @@ class _Emitter:
                 ds.add(tuple(-d for d in r.dmpi))
         return sorted(ds)
 
-    def layout(self) -> PlannedLayout:
-        return PlannedLayout(self.lay, [-x for x in self.static_box.lo], self.static_box.shape())
+    def layout(self) -> Layout:
+        return PlannedLayout(self.lay, [-x for x in self.static_box.lo], self.static_box.shape(), self.deltas())
 
     def header(self) -> str:
-        lay = self.layout()
-        dims = "".join(f"[{s}]" for s in lay.shape())
-        out = [f"/* {lay.name}.h -- Formura_Navigator ABI (region-plan oracle). */", "#pragma once",
-               "#ifdef __cplusplus", 'extern "C"', "{", "#endif", "#include <mpi.h>", ""]
-        for a, n in zip(lay.axes, lay.intra):
-            out.append(f"#define N{a.upper()}  {n}")
-        for f, t in zip(lay.field_names, lay.field_ctype):
-            out.append(f"extern {t} {f}{dims};")
-        out.append("struct Formura_Navigator {")
-        out.append("int time_step;")
-        for a in lay.axes:
-            out += [f"int lower_{a};", f"int upper_{a};", f"int offset_{a};"]
-        out += ["MPI_Comm mpi_comm;", "int mpi_my_rank;"]
-        for d in self.deltas():
-            out.append(f"int {delta_field(d)};")
-        out += ["};", "extern struct Formura_Navigator navi;",
-                "int Formura_Init (struct Formura_Navigator *navi, MPI_Comm comm);",
-                "int Formura_Forward (struct Formura_Navigator *navi);", "", "#ifdef __cplusplus", "}", "#endif", ""]
-        return "\n".join(out)
+        return header_text(self.layout())
 
     def source(self) -> str:
         lay = self.layout()

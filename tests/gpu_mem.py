@@ -51,3 +51,11 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def download(ptr: int, shape, dtype=np.float64) -> np.ndarray:
+    """Copy `shape` elements of `dtype` from device address `ptr` to the host."""
+    out = np.empty(shape, dtype=dtype)
+    if _rt().hipMemcpy(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes, 2) != 0:   # hipMemcpyDeviceToHost
+        raise RuntimeError("hipMemcpy failed")
+    return out

@@ -1,4 +1,4 @@
-"""The region plan restated from Cut.hs (oracle/cut_plan.py) against the
+"""The region plan restated from Cut.hs (formura_amd/compiler/cut_plan.py) against the
 reference's own verbose plan print for examples-generated/pearson.fmr
 (examples-generated/pearson.log:85-466: allocation list, buffer sharing,
 ridge list and distributed program of the normal and the wall-inverted plan;
@@ -16,7 +16,7 @@ from formura_amd.compiler.manifest import gen_mm_program
 from formura_amd.compiler.numerical_config import NumericalConfig
 from formura_amd.compiler.omgraph import gen_om_program
 from formura_amd.compiler.parser import parse_program
-from oracle.cut_plan import Box, IRank, Rsc, RidgeID, make_plan
+from formura_amd.compiler.cut_plan import Box, IRank, Rsc, RidgeID, make_plan
 
 
 def vec(v):

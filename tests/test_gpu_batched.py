@@ -68,6 +68,37 @@ def test_fp64_batch_every_variant_matches_oracle_bitwise():
                     f"instance {i} differs for T={T} vy={vy} lx={lx}"
 
 
+def test_fused_launch_mixed_lane_widths_matches_oracle_bitwise():
+    """A plan of one depth T with a different lane width and march chunk per
+    bucket runs as one persistent launch per T-block over the tiles of every
+    instance (Formura_Batch_Fused); the captured graph holds one kernel node
+    per T-block; the FP64 result equals the oracle bit for bit."""
+    lib = batch("gs2d_batch_f64", SMALL)
+    ics = [initial_state(s, 200 + i) for i, s in enumerate(SMALL)]
+    want = [oracle_after(i, u, v, 1) for i, (u, v) in enumerate(ics)]
+    steps = want[0][2]
+    widths = {}
+    for T, vy in lib.variants():
+        widths.setdefault(T, set()).add(vy)
+    for T in (1, 2, 4, 8):
+        if T not in widths or len(widths[T]) < 2 or steps % T:
+            continue
+        a, b = sorted(widths[T])[:2]
+        lib.plan([0, 1, 1], [(T, a, 16), (T, b, 64)])
+        assert lib.fused() == (T, (1 if a == 1 else 2 if a == 2 else 4) | (1 if b == 1 else 2 if b == 2 else 4))
+        for graph in (False, True):
+            _say(f"[batched] fused T={T} widths {a},{b} graph={graph} steps={steps}")
+            for i, (u, v) in enumerate(ics):
+                lib.set_state(i, [u, v])
+            lib.advance(steps, graph=graph)
+            if graph:
+                assert lib.last_graph_nodes() == steps // T
+            for i in range(len(SMALL)):
+                gu, gv = lib.get_state(i)
+                assert np.array_equal(gu, want[i][0]) and np.array_equal(gv, want[i][1]), \
+                    f"instance {i} differs for fused T={T} widths {a},{b} graph={graph}"
+
+
 def test_graph_replays_match_eager_launches():
     lib = batch("gs2d_batch_f64", SMALL)
     ics = [initial_state(s, 7 + i) for i, s in enumerate(SMALL)]

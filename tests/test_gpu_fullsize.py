@@ -3,15 +3,16 @@ can pick: each march variant compiled into the 512^3 libraries, with each
 work split of runtime.march_chunks, and the 64^3 Gray-Scott program over its
 full 1000 steps.
 
-* FP64 (heat 512^3, Gray-Scott 512^3): 10 steps of each (variant, split)
-  against the CPU oracle's 10 steps.  The oracle's result is uploaded once
+* FP64 (heat 512^3, Gray-Scott 512^3): 10 (heat) or 12 (Gray-Scott, the
+  monitor interval rounded up to a multiple of 2T) steps of each (variant,
+  split) against the CPU oracle's same steps.  The oracle's result is uploaded once
   and compared on the device: max |GPU - oracle| must be 0 and the sum of
   squared differences 0, i.e. every site equal.  The steps left when 10 is
   not a multiple of T run on a variant of the deepest depth that fits, as in
   any Formura_Forward_Device call.
 * FP32 bulk (heat 512^3, Gray-Scott 512^3): every (variant, split) against
   the FP32 direct kernel (one site per thread, neighbours from global
-  memory) over the same 10 steps, bit for bit -- the march kernels evaluate
+  memory) over the same steps, bit for bit -- the march kernels evaluate
   the same float operations in the same order -- and the direct kernel's
   result against the FP64 oracle within the stated tolerance.
 * Gray-Scott 64^3 FP64 (the reference's CPU-runnable case): 1000 steps of
@@ -32,12 +33,11 @@ import numpy as np
 import pytest
 
 from formura_amd import build
-from formura_amd.runtime import KIND_DIRECT, KIND_MARCH, StepLibrary, load_program, march_chunks
+from formura_amd.runtime import KIND_DIRECT, StepLibrary, load_program, march_chunks
 from tests.common import gray_scott_ic, normal
 
 pytestmark = pytest.mark.gpu
 
-STEPS = 10
 
 
 def _say(msg):
@@ -46,7 +46,7 @@ def _say(msg):
 
 
 def compiled_variants(name):
-    """(T, rows, waves, pf, occ) of the march variants compiled into
+    """(kind, T, rows, waves, pf, occ) of the march variants compiled into
     lib<name>.so, from the build's resource file (no GPU needed to list)."""
     path = build.lib_path(name)[:-3] + ".resources.json"
     if not os.path.exists(path):
@@ -74,20 +74,25 @@ def state_for(lib, seed):
     return {"U": u, "V": v}
 
 
+def oracle_steps(oracle_name):
+    from oracle.build_oracle import compiled_extra
+    return compiled_extra(oracle_name).layout.steps_per_forward
+
+
 def oracle_state(oracle_name, st):
-    """The CPU oracle's result after one Formura_Forward (STEPS steps)."""
+    """The CPU oracle's result after one Formura_Forward (oracle_steps)."""
     from oracle.build_oracle import compiled_extra, oracle_path
     c = compiled_extra(oracle_name)
     o = StepLibrary(oracle_path(oracle_name), c.layout, device=False)
     o.init()
-    assert c.layout.steps_per_forward == STEPS
     o.set_state(st)
     o.forward()
     return o.get_state()
 
 
-def reference(key, make):
+def reference(name, make):
     """Device copies of a reference result, computed once per module."""
+    key = name + "/ref"
     if key not in _cache:
         from tests.gpu_mem import DeviceArray
         st, want = make()
@@ -95,7 +100,7 @@ def reference(key, make):
     return _cache[key]
 
 
-def run_and_compare(lib, st, dev_ref, label, steps=STEPS):
+def run_and_compare(lib, st, dev_ref, label, steps):
     lib.set_state(st)
     lib.upload()
     lib.forward_device(steps)
@@ -125,19 +130,20 @@ def kernel_runs(variant):
     """(kernel, split) pairs of one variant: the direct kernel for None."""
     if variant is None:
         return [((KIND_DIRECT, 1, 0, 1, 1, 1), 64)]
-    return [((KIND_MARCH,) + tuple(variant), lx) for lx in march_chunks(variant[4])]
+    return [(tuple(variant), lx) for lx in march_chunks(variant[5])]
 
 
 @pytest.mark.parametrize("name,oname,seed,variant", _fp64_params())
 def test_fp64_variant_full_size_bitwise(name, oname, seed, variant):
     lib = hip(name)
     st, dev_ref, _ = reference(name, lambda: _fp64_reference(lib, oname, seed))
+    steps = oracle_steps(oname)
     for var, lx in kernel_runs(variant):
-        label = f"{name} kernel {var} lx={lx} {STEPS} steps"
+        label = f"{name} kernel {var} lx={lx} {steps} steps"
         _say(f"[fullsize] {label}")
         lib.set_kernel(*var)
         lib.set_chunk(lx)
-        run_and_compare(lib, st, dev_ref, label)
+        run_and_compare(lib, st, dev_ref, label, steps)
     lib.set_chunk(64)
 
 
@@ -145,20 +151,20 @@ FP32_CASES = [("heat512_f32", "heat512_cpu", 513, 1e-6), ("gs3d512_f32", "gs3d51
 
 
 def _fp32_reference(name, oname, seed, tol):
-    """The FP32 direct kernel's result after STEPS steps; checked against the
+    """The FP32 direct kernel's result after the oracle's steps; checked against the
     FP64 oracle within `tol` (relative L2 per field)."""
     lib = hip(name)
     st = state_for(lib, seed)
     lib.set_kernel(KIND_DIRECT)
     lib.set_state(st)
     lib.upload()
-    lib.forward_device(STEPS)
+    lib.forward_device(oracle_steps(oname))
     lib.download()
     got = lib.get_state()
     want = oracle_state(oname, st)
     for f in want:
         err = np.linalg.norm(got[f] - want[f]) / np.linalg.norm(want[f])
-        _say(f"[fullsize] {name} direct FP32 vs FP64 oracle, field {f}: rel L2 {err:.3e} after {STEPS} steps")
+        _say(f"[fullsize] {name} direct FP32 vs FP64 oracle, field {f}: rel L2 {err:.3e}")
         assert err <= tol, f"{name}: FP32 direct kernel rel L2 {err:.3e} > {tol}"
     return st, {f: got[f].astype(np.float32) for f in got}
 
@@ -175,12 +181,13 @@ def _fp32_params():
 def test_fp32_variant_full_size_equals_direct(name, oname, seed, tol, variant):
     lib = hip(name)
     st, dev_ref, _ = reference(name, lambda: _fp32_reference(name, oname, seed, tol))
+    steps = oracle_steps(oname)
     for var, lx in kernel_runs(variant):
-        label = f"{name} kernel {var} lx={lx} {STEPS} steps vs FP32 direct"
+        label = f"{name} kernel {var} lx={lx} {steps} steps vs FP32 direct"
         _say(f"[fullsize] {label}")
         lib.set_kernel(*var)
         lib.set_chunk(lx)
-        run_and_compare(lib, st, dev_ref, label)
+        run_and_compare(lib, st, dev_ref, label, steps)
     lib.set_chunk(64)
 
 

@@ -204,3 +204,54 @@ def test_scatter_and_gather_round_trip():
     assert np.array_equal(bback.download((nptr, n)), src)
     for b in parts + [bsrc, bback]:
         b.free()
+
+
+def test_accumulated_check_measures_the_drift():
+    """The accumulated-error check of a checked rollout: shadow instances,
+    advanced by the FP32 stencil alone from the rollout's start, against the
+    rollout's state of the same trajectories after each window.  Its last
+    value must equal (to 1e-6 relative) the host-computed relative L2
+    between the rollout's final state of those trajectories and an FP32
+    stencil run of them for the same steps; and the rollout either keeps
+    every accumulated error within the bound or falls back at the window
+    where one exceeds it."""
+    from formura_amd import surrogate as S
+    theta, scale, _ = S.load_weights()
+    sur = S.Surrogate.get()
+    specs = S.trajectory_specs(8, seed=99)
+    x0 = S.initial_states(specs, 99000)
+    st = S.Stencil(specs)
+    for i in range(len(specs)):
+        st.lib.set_state(i, [x0[i, 0].astype(np.float64), x0[i, 1].astype(np.float64)])
+    st.advance(256)
+    buf = sur.buffer(x0.nbytes)
+    st.store(sur, buf)
+    xs = buf.download(x0.shape)
+    fk = sur.buffer(8 * len(specs))
+    fk.upload(S.fk_array(specs))
+    sur.set_params(theta)
+    sur.set_scale(*scale)
+    shadows = S.shadow_sample(len(specs))
+    n_super = 16
+    sts = S.Stencil(specs, shadows=shadows)
+    ro = S.checked_rollout(sur, sts, fk, xs, n_super)
+    _say(f"[surrogate] accumulated errors {ro.accumulated}, local {ro.checks}, fallback {ro.fallback_at}")
+    assert len(ro.accumulated) == len(ro.checks)
+    if ro.fallback_at is None:
+        assert max(ro.accumulated) <= S.BOUND
+        # the stencil alone, from the same start, for the shadowed trajectories
+        st2 = S.Stencil(specs)
+        buf.upload(xs)
+        st2.load(sur, buf)
+        st2.advance(S.SUPER * n_super)
+        st2.store(sur, buf)
+        alone = buf.download(x0.shape)
+        host = max(S.rel_l2(ro.state[i].astype(np.float64), alone[i].astype(np.float64)) for i in shadows)
+        _say(f"[surrogate] last accumulated error {ro.accumulated[-1]:.6e}, host {host:.6e}")
+        assert abs(ro.accumulated[-1] - host) <= 1e-6 * host + 1e-12
+    else:
+        w = ro.fallback_at // S.CHECK_EVERY
+        assert ro.accumulated[w] > S.BOUND or ro.checks[w] > S.BOUND
+        assert all(a <= S.BOUND for a in ro.accumulated[:w])
+    for b in (buf, fk):
+        b.free()
