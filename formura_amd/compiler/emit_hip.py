@@ -566,6 +566,13 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
     w("  return 0;")
     w("}")
     w("")
+    w("int Formura_Diff_Arrays_NoAlloc (const void *a, int a_elem_size, const void *ref, int ref_elem_size,")
+    w("                                 long long n, double *res, void *stream)")
+    w("{")
+    w("  return fmr::diff_noalloc_entry(a, a_elem_size, ref, ref_elem_size, n, res,")
+    w("                                 stream ? (hipStream_t)stream : lib.stream);")
+    w("}")
+    w("")
     w('}  // extern "C"')
     return "\n".join(out) + "\n"
 

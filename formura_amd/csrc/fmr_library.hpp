@@ -72,6 +72,73 @@ __global__ void __launch_bounds__(64) reduce_pass2(const double* part, double* r
   if (threadIdx.x == 0) { res[0] = t0; res[1] = t1; res[2] = t2; }
 }
 
+// ---- the same comparison without any allocation ---------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops): the per-block partial sums live in `scratch`, a
+// file-scope device array of 3 * RED_BLOCKS doubles (24 KiB) that each GPU
+// the library runs on holds until the process ends.  Every block writes its
+// own three partials on every call (also a block with no element: zeros), so
+// nothing needs clearing between calls; calls that overlap on one GPU race
+// on the array.  In host code `scratch` names a host-side stand-in, so the
+// device address is looked up once per device (hipGetSymbolAddress) and only
+// that address is handed to kernels.
+constexpr long long NOALLOC_MAX_ELEMS = 1LL << 36;   // elements per array
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[3 * RED_BLOCKS];
+
+struct ScratchTable {
+  double* ptr[NOALLOC_MAX_DEVICES] = {};
+
+  // the current device's copy of `scratch`, looked up on first use; null on error
+  double* current() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    if (!ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) {
+        fprintf(stderr, "formura_amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+        return nullptr;
+      }
+      ptr[dev] = (double*)p;
+    }
+    return ptr[dev];
+  }
+};
+
+inline ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// res (device memory, 3 doubles) = {sum (a - ref)^2, sum ref^2, max |a - ref|}
+// over n elements, enqueued on `s`; the kernels and their order are those of
+// StepLibrary::diff, so a whole field gives the same bits.  Returns 0, or -1
+// for a bad argument (nothing is launched then).
+template <class A, class R>
+int diff_noalloc(const A* a, const R* ref, long long n, double* res, hipStream_t s) {
+  double* part = scratch_table().current();
+  if (!part) return -1;
+  hipLaunchKernelGGL((reduce_pass1<A, R, 1>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, a, ref, (size_t)n, part);
+  hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(64), 0, s, (const double*)part, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+inline int diff_noalloc_entry(const void* a, int a_elem, const void* ref, int ref_elem, long long n, double* res,
+                              hipStream_t s) {
+  if (!a || !ref || !res || n < 0) return -1;
+  if (n > NOALLOC_MAX_ELEMS) {
+    fprintf(stderr, "formura_amd: Formura_Diff_Arrays_NoAlloc: %lld elements exceed the maximum %lld\n", n,
+            NOALLOC_MAX_ELEMS);
+    return -1;
+  }
+  if (a_elem == 8 && ref_elem == 8) return diff_noalloc((const double*)a, (const double*)ref, n, res, s);
+  if (a_elem == 8 && ref_elem == 4) return diff_noalloc((const double*)a, (const float*)ref, n, res, s);
+  if (a_elem == 4 && ref_elem == 8) return diff_noalloc((const float*)a, (const double*)ref, n, res, s);
+  if (a_elem == 4 && ref_elem == 4) return diff_noalloc((const float*)a, (const float*)ref, n, res, s);
+  return -1;
+}
+
 // ---- kernel variants --------------------------------------------------------
 // KIND_MARCH: march3d_kernel (fmr_stencil.hpp); KIND_MARCHV1 / KIND_MARCHV2:
 // marchv_kernel (fmr_march.hpp) with one / two z-sites per lane
@@ -235,6 +302,7 @@ class StepLibrary {
     FMR_CHECK(hipEventCreate(&ev0));
     FMR_CHECK(hipEventCreate(&evm));
     FMR_CHECK(hipEventCreate(&ev1));
+    scratch_table().current();   // the no-allocation entry point's scratch on this device
     // default: the fused variant with T = 2 if compiled in, else the first one
     // (programs without an inlined point function: the stage kernels, set by
     // init_stages)

@@ -109,6 +109,21 @@ int Formura_Reduce_Device(int f, double *res);
  * res[2] = max |f - ref|.  Returns 0 or -1. */
 int Formura_Diff_Device(int f, const void *ref, int ref_elem_size, double *res);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the comparison of Formura_Diff_Device, for two caller-owned
+ * device arrays a and ref of n elements each (element sizes 8 or 4),
+ * res (DEVICE memory, 3 doubles) = {sum (a - ref)^2, sum ref^2,
+ * max |a - ref|}, enqueued on `stream` (a hipStream_t; NULL: the library's
+ * stream) and not waited for.  It allocates nothing: its partial sums live
+ * in a static 24 KiB device array of the library, one per GPU, held until
+ * the process ends, so calls that overlap on one GPU race.  Same kernels
+ * and summation order as Formura_Diff_Device: the same bits for the same
+ * data.  Returns 0, or -1 for a null pointer, an element size other than 4
+ * or 8, n < 0 or n > 2^36 (nothing is enqueued then). */
+int Formura_Diff_Arrays_NoAlloc(const void *a, int a_elem_size, const void *ref, int ref_elem_size, long long n,
+                                double *res, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
