@@ -50,6 +50,9 @@ def seeded_state(lib, seed):
     shape = tuple(lib.layout.intra)
     if lib.layout.field_names == ["u"]:
         return {"u": rng.standard_normal(shape)}
+    if len(lib.layout.field_names) == 9:
+        from formura_amd.rng import mhd_state
+        return mhd_state(shape, seed)
     if lib.layout.field_names == ["r", "mx", "my", "mz"]:
         r = 1.0 + 0.2 * normal(seed, shape)
         return {"r": r, "mx": r * 0.1 * normal(seed + 1, shape), "my": r * 0.1 * normal(seed + 2, shape),
@@ -157,21 +160,25 @@ def _time_cpu_library(lib, cells, per, sample_steps):
 
 
 def cpu_baseline(sample_steps=96, plain_steps=20):
-    """Heat 512^3 FP64 on the host cores, about `sample_steps` steps of the
-    region-plan library and `plain_steps` of the plain oracle: the C generated on the reference's own region plan
-    (Cut.hs restated: 2 x 2 x 2 regions, T = 2, skewed in-place stores, ridges
-    and self-message facets) and the plain oracle (one loop nest per manifest
-    node over the whole grid).  The faster one is the baseline."""
+    """Heat 512^3 FP64 on the host cores, about `sample_steps` steps of each
+    region-plan library and `plain_steps` of the plain oracle: the C generated
+    on the reference's own region plan (Cut.hs restated: skewed in-place
+    stores, ridges and self-message facets) with 2 x 2 x 2 regions at T = 2,
+    and with the option shape of the reference's examples/pearson-3d.yaml
+    (T = 8, 2-site nanoblocks along x), and the plain oracle (one loop nest
+    per manifest node over the whole grid).  The fastest one is the
+    baseline."""
     import numpy as np
     from oracle.build_oracle import compiled_extra, oracle_path, planned_layout, planned_path
     from formura_amd.runtime import StepLibrary
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     runs = {}
-    c, lay = planned_layout("heat512_plan")
-    lib = StepLibrary(planned_path("heat512_plan"), lay, device=False)
-    lib.init()
-    cells = int(np.prod(lay.intra))
-    runs["region plan"] = _time_cpu_library(lib, cells, lay.steps_per_forward, sample_steps)
+    for name, label in (("heat512_plan", "region plan T=2"), ("heat512_t8_plan", "region plan T=8 nbux2")):
+        c, lay = planned_layout(name)
+        lib = StepLibrary(planned_path(name), lay, device=False)
+        lib.init()
+        cells = int(np.prod(lay.intra))
+        runs[label] = _time_cpu_library(lib, cells, lay.steps_per_forward, sample_steps)
     c2 = compiled_extra("heat512_cpu")
     lib2 = StepLibrary(oracle_path("heat512_cpu"), c2.layout, device=False)
     lib2.init()
@@ -179,8 +186,10 @@ def cpu_baseline(sample_steps=96, plain_steps=20):
     best = max(runs, key=lambda k: runs[k][0])
     rate, steps, dt = runs[best]
     other = [f"{k} {v[0]:.3f} GLUP/s" for k, v in runs.items() if k != best]
-    what = ("C on the reference's region plan (Cut.hs restated: 2x2x2 regions, T=2)" if best == "region plan"
-            else "plain oracle C (one loop nest per manifest node)")
+    what = {"region plan T=2": "C on the reference's region plan (Cut.hs restated: 2x2x2 regions, T=2)",
+            "region plan T=8 nbux2": "C on the reference's region plan with the options of its pearson-3d.yaml "
+                                     "(T=8, nbux2, 2x2x2 regions)",
+            "plain oracle": "plain oracle C (one loop nest per manifest node)"}[best]
     return {"value": round(rate, 4), "unit": "GLUP/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of heat 512^3 FP64, {what}, gcc -O3 -fopenmp -ffp-contract=off, "
                       f"{dt:.1f} s; also timed: {'; '.join(other)}"}
@@ -190,7 +199,7 @@ def cpu_baseline(sample_steps=96, plain_steps=20):
 EXTRA_PROGRAMS = (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp64"),
                   ("gs3d512_f32", "grayscott512_fp32"), ("gs3d64", "grayscott64_fp64"),
                   ("heat_rk2", "heat256_midpoint_fp64"), ("gs3d_rk4", "grayscott64_rk4_fp64"),
-                  ("fveuler", "fv_euler256_ssprk3_fp64"))
+                  ("fveuler", "fv_euler256_ssprk3_fp64"), ("mhd", "mhd_glm128_ssprk3_fp64"))
 
 
 def program_extras(only=None, libs=None):

@@ -48,3 +48,18 @@ def gray_scott_ic(shape, seed: int, cube_frac: float = 0.125, noise: float = 0.0
     u += noise * (2.0 * uniform(seed, shape) - 1.0)
     v += noise * (2.0 * uniform(seed + 1, shape) - 1.0)
     return u, v
+
+
+def mhd_state(shape, seed: int):
+    """Fields of the MHD program (examples/mhd-glm-3d.fmr): density
+    1 + 0.1 N(0,1) (positive), momentum r * 0.1 N(0,1) per axis, field
+    0.3 N(0,1) per axis, scalar density r * U[0,1), cleaning potential 0."""
+    r = 1.0 + 0.1 * normal(seed, shape)
+    out = {"r": r}
+    for k, f in enumerate(("mx", "my", "mz")):
+        out[f] = r * 0.1 * normal(seed + 1 + k, shape)
+    for k, f in enumerate(("bx", "by", "bz")):
+        out[f] = 0.3 * normal(seed + 4 + k, shape)
+    out["rs"] = r * uniform(seed + 7, shape)
+    out["psi"] = 0.0 * r
+    return out

@@ -63,6 +63,15 @@ PLANNED = {
                                      "initial_walls": {"x": [16], "y": [20], "z": [18]}}, 2),
     "gs3d_rk4_plan": ("gs3d_rk4_t", {"intra_node_shape": [40, 48, 56], "monitor_interval": 4,
                                      "initial_walls": {"x": [20], "y": [24], "z": [28]}}, 2),
+    # nanoblock unroll (nbu<axis>N): two programs with 2-site nanoblocks
+    "gs3d_nbu_plan": ("gs3d_t", {"monitor_interval": 4, "initial_walls": {"x": [12], "y": [40], "z": [64, 100]},
+                                 "option_strings": ["omp", "omp-collapse", "nbux2", "nbuz2"]}, 2),
+    "heat_nbu_plan": ("heat_t48", {"monitor_interval": 16, "initial_walls": {"x": [20], "y": [24], "z": [36]},
+                                   "option_strings": ["omp", "omp-collapse", "nbux2"]}, 8),
+    # the option shape of the reference's examples/pearson-3d.yaml (T = 8,
+    # nbux2, one wall per axis) at the bench's heat 512^3: a second CPU baseline
+    "heat512_t8_plan": ("heat512", {"monitor_interval": 16, "initial_walls": {"x": [272], "y": [272], "z": [272]},
+                                    "option_strings": ["omp", "omp-collapse", "nbux2"]}, 8),
 }
 
 

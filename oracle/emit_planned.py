@@ -23,6 +23,12 @@ turned into C as the reference's MPI C++ backend does
   * `Formura_Forward` runs monitorInterval2 / (2T) times the normal plan then
     the inverted one, over the T-fold step graph (:1106-1131).
 
+  * nanoblock unroll `nbu<axis>N` (Manifestation.hs:146-158, :698-719): the
+    loops step by the nanoblock, one iteration computes every cursor of it
+    (an operation shared by several cursors once) and stores each; regions
+    that are not whole nanoblocks, where the reference's loop would run past
+    the region, are refused.
+
 Names follow the reference's patterns (`Rsc<k>`, `Ridge_...`, `Facet_...`,
 `req_...`) but are not guaranteed to be its exact text; the loop-template
 deduplication into `Formura_subroutine_N` (a code-size measure) is left out.
@@ -83,6 +89,25 @@ def delta_field(d) -> str:
     return delta_name(d)
 
 
+def _nbu_size(options, axis: str) -> int:
+    """Nanoblock unroll `nbu<axis>N` (NumericalConfig.hs:48-53): the first
+    option with that prefix, else 1."""
+    kwd = "nbu" + axis
+    for opt in options:
+        if opt.startswith(kwd):
+            return int(opt[len(kwd):])
+    return 1
+
+
+def _nanoblock_cursors(nbu) -> List[tuple]:
+    """Cursors of one nanoblock in the order the reference manifests them
+    (Manifestation.hs:146-158: x outer, z inner, then reversed)."""
+    cs = [()]
+    for n in nbu:
+        cs = [c + (k,) for c in cs for k in range(n)]
+    return list(reversed(cs))
+
+
 class _Emitter:
     def __init__(self, lay: Layout, mm: MMProgram, T: int, walls, options):
         self.lay = lay
@@ -102,7 +127,8 @@ class _Emitter:
             for k, b in p.array_alloc.items():
                 if k.static:
                     hull = b if hull is None else hull.hull(b)
-        self.static_box = Box(hull.lo, tuple(u + 2 for u in hull.hi))
+        self.nbu = [_nbu_size(self.options, a) for a in self.axes]
+        self.static_box = Box(hull.lo, tuple(u + n - 1 + 2 for u, n in zip(hull.hi, self.nbu)))
         for p in self.plans:
             for k in list(p.array_alloc):
                 if k.static:
@@ -149,53 +175,86 @@ class _Emitter:
         to = [a - b for a, b in zip(region.hi, margin.lo)]
         ind = margin.lo
         n = self.step[nid]
-        names: Dict[int, str] = {}
+        # nanoblock unroll (Manifestation.hs:146-158, :698-719): one loop
+        # iteration computes the sites i + c for every cursor c of the
+        # nanoblock, a micro node shared by several cursors (the same
+        # operation on the same shifted operands) once, and stores each
+        # cursor's root value
+        nbu = self.nbu
+        cursors = _nanoblock_cursors(nbu) if any(k > 1 for k in nbu) else [(0,) * self.dim]
+        if len(cursors) > 1:
+            if any((h - l) % k for l, h, k in zip(frm, to, nbu)):
+                raise ValueError(f"region {region} is not a whole number of nanoblocks {nbu}: the reference's "
+                                 "loop would run past the region")
+            if any(mn.inst[0] == "LoadIndex" for mn in n.micro.values()):
+                raise ValueError("index variables under nanoblock unroll are not restated")
+        names: Dict[tuple, str] = {}
         lines: List[str] = []
-        for j in sorted(n.micro):
-            mn = n.micro[j]
-            inst = mn.inst
-            tag = inst[0]
-            ct = _ctype(mn.elem)
-            if tag == "LoadCursorStatic":
-                ab = p.array_alloc[Rsc(True, inst[2])]
-                names[j] = inst[2] + self.acc([i + v - l for i, v, l in zip(ind, inst[1], ab.lo)])
-                continue
-            if tag == "LoadCursor":
-                key = Rsc(False, inst[2], ir)
-                src = self.step[inst[2]]
-                if src.typ[0] != "grid":
-                    names[j] = self.rsc_name(p, key)
+        roots: List[tuple] = []
+        seen: Dict[tuple, str] = {}
+        for c in cursors:
+            key_of: Dict[int, tuple] = {}
+            for j in sorted(n.micro):
+                mn = n.micro[j]
+                inst = mn.inst
+                tag = inst[0]
+                ct = _ctype(mn.elem)
+                if tag in ("LoadCursorStatic", "LoadCursor"):
+                    vc = tuple(v + d for v, d in zip(inst[1], c))
+                    key = (tag, vc, inst[2])
+                elif tag == "Imm":
+                    key = (tag, inst[1], mn.elem)
+                elif tag == "Store":
+                    key_of[j] = key_of[inst[2]]
+                    continue
+                elif tag == "LoadIndex":
+                    key = (tag, inst[1], c)
                 else:
-                    ab = p.array_alloc[key]
-                    names[j] = self.rsc_name(p, key) + self.acc([i + v - l for i, v, l in zip(ind, inst[1], ab.lo)])
-                continue
-            if tag == "Imm":
-                names[j] = imm_literal(inst[1], "double")
-                continue
-            if tag == "Store":
-                names[j] = names[inst[2]]
-                continue
-            if tag == "LoadIndex":
-                ax = inst[1]
-                code = f"(navi.offset_{self.axes[ax]}+{self.ivars[ax]}+({ind[ax]}))"
-            else:
-                kind = {"Uniop": "uniop", "Binop": "binop", "Triop": "triop", "Naryop": "naryop"}[tag]
-                code = c_expr_of(Op(kind, inst[1:], ct), lambda i: names[i])
-            v = f"a_{j}"
-            lines.append(f"{ct} {v} = {code};")
-            names[j] = v
-        root = names[n.root()]
-        if dest.static:
-            lhs = dest.name + self.acc([-s for s in p.system_offset])
-        else:
-            lhs = self.rsc_name(p, dest) + self.acc([0] * self.dim)
+                    args = tuple(key_of[a] if isinstance(a, int) else tuple(key_of[x] for x in a)
+                                 for a in inst[2:])
+                    key = (tag, inst[1], args, mn.elem)
+                key_of[j] = key
+                if key in seen:
+                    continue
+                if tag == "LoadCursorStatic":
+                    ab = p.array_alloc[Rsc(True, inst[2])]
+                    seen[key] = inst[2] + self.acc([i + v - l for i, v, l in zip(ind, key[1], ab.lo)])
+                    continue
+                if tag == "LoadCursor":
+                    rk = Rsc(False, inst[2], ir)
+                    src = self.step[inst[2]]
+                    if src.typ[0] != "grid":
+                        seen[key] = self.rsc_name(p, rk)
+                    else:
+                        ab = p.array_alloc[rk]
+                        seen[key] = self.rsc_name(p, rk) + self.acc([i + v - l for i, v, l in zip(ind, key[1], ab.lo)])
+                    continue
+                if tag == "Imm":
+                    seen[key] = imm_literal(inst[1], "double")
+                    continue
+                if tag == "LoadIndex":
+                    ax = inst[1]
+                    code = f"(navi.offset_{self.axes[ax]}+{self.ivars[ax]}+({ind[ax]}))"
+                else:
+                    kind = {"Uniop": "uniop", "Binop": "binop", "Triop": "triop", "Naryop": "naryop"}[tag]
+                    code = c_expr_of(Op(kind, inst[1:], ct), lambda i: seen[key_of[i]])
+                v = f"a_{len(lines)}"
+                lines.append(f"{ct} {v} = {code};")
+                seen[key] = v
+            roots.append((c, seen[key_of[n.root()]]))
         out = []
         if "omp-collapse" in self.options and self.dim > 1:
             out.append(f"#pragma omp for collapse({self.dim - 1})")
         for a in range(self.dim):
-            out.append(f"for (int {self.ivars[a]} = {frm[a]}; {self.ivars[a]} < {to[a]}; {self.ivars[a]} += 1) {{")
+            out.append(f"for (int {self.ivars[a]} = {frm[a]}; {self.ivars[a]} < {to[a]}; "
+                       f"{self.ivars[a]} += {nbu[a]}) {{")
         out += lines
-        out.append(f"{lhs} = {root};")
+        for c, root in roots:
+            if dest.static:
+                lhs = dest.name + self.acc([d - s for d, s in zip(c, p.system_offset)])
+            else:
+                lhs = self.rsc_name(p, dest) + self.acc(list(c))
+            out.append(f"{lhs} = {root};")
         out += ["}"] * self.dim
         return out
 
@@ -368,8 +427,6 @@ def write_planned(outdir: str, lay: Layout, mm: MMProgram, T: int = 1, walls=Non
             walls[a] = [n // 2]
     if options is None:
         options = list(getattr(nc, "option_strings", None) or ["omp", "omp-collapse"])
-    if nc is not None and any(nc.nbu_size(a) != 1 for a in "xyz"):
-        raise ValueError("nanoblock unroll is not restated by the region-plan oracle")
     e = _Emitter(lay, mm, T, walls, options)
     src = e.source()
     os.makedirs(outdir, exist_ok=True)

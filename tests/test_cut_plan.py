@@ -128,7 +128,8 @@ def _states(name, shape):
     return {"U": u, "V": v}
 
 
-@pytest.mark.parametrize("name", ["gs3d_plan", "heat_plan", "gs2d_plan", "heat_rk2_plan", "gs3d_rk4_plan"])
+@pytest.mark.parametrize("name", ["gs3d_plan", "heat_plan", "gs2d_plan", "heat_rk2_plan", "gs3d_rk4_plan",
+                                  "gs3d_nbu_plan", "heat_nbu_plan"])
 def test_region_plan_library_equals_oracle_bitwise(name, tmp_path):
     """The C generated on the restated region plan (regions, skewed in-place
     stores, ridges and self-message facets, normal and inverted plans, the
@@ -180,3 +181,41 @@ def test_walls_closer_than_the_reach_are_rejected():
     c = compile_source("narrow", src, nc)
     with pytest.raises(ValueError, match="narrower than the stencil reach"):
         _Emitter(c.layout, c.mm, 2, walls, ["omp"])
+
+
+def test_nanoblock_unroll_loops_and_margins():
+    """Nanoblock unroll (Manifestation.hs:146-158, MPICxx/Translate.hs:698-719,
+    :942): with nbux2 every loop nest steps x by 2 and stores two sites per
+    iteration, an operation that both sites use is computed once, and the
+    state arrays grow by nbu - 1 on the upper side of x only.  A region that
+    is not a whole number of nanoblocks is refused."""
+    from formura_amd.compiler.driver import compile_source
+    from oracle.emit_planned import _Emitter
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "examples", "heat-3d-512.fmr"), encoding="utf-8").read()
+    walls = {"x": [10], "y": [12], "z": [14]}
+
+    def emitter(opts, shape=(20, 24, 28)):
+        nc = NumericalConfig(intra_node_shape=list(shape), mpi_grid_shape=[1, 1, 1], temporal_blocking_interval=2,
+                             monitor_interval=4, initial_walls=walls, option_strings=opts)
+        c = compile_source("nb", src, nc)
+        return _Emitter(c.layout, c.mm, 2, walls, opts)
+
+    plain = emitter(["omp"])
+    nb = emitter(["omp", "nbux2"])
+    assert [u - p for u, p in zip(nb.static_box.hi, plain.static_box.hi)] == [1, 0, 0]
+    assert nb.static_box.lo == plain.static_box.lo
+    p = plain.plans[0]
+    inst = next(i for i in p.program if i[0] == "compute")
+    a, b = plain.computation(p, *inst[1], inst[2]), nb.computation(nb.plans[0], *inst[1], inst[2])
+    assert "ix += 2)" in "\n".join(b) and "ix += 1)" in "\n".join(a)
+    stores = lambda code: [ln for ln in code if " = a_" in ln and not ln.startswith(("double", "float"))]
+    ops = lambda code: [ln for ln in code if ln.startswith("double a_")]
+    assert len(stores(b)) == 2 * len(stores(a)) == 2
+    assert len(ops(a)) < len(ops(b)) < 2 * len(ops(a))
+    with pytest.raises(ValueError, match="whole number of nanoblocks"):
+        e = emitter(["omp", "nbux3"])
+        q = e.plans[0]
+        for i in q.program:
+            if i[0] == "compute":
+                e.computation(q, *i[1], i[2])

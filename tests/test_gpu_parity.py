@@ -290,6 +290,27 @@ def test_large_multistage_program_bitwise():
     assert_bitwise(lib.get_state(), oracle_run("fveuler_t", st, 1), "fveuler_t Formura_Forward")
 
 
+def test_mhd_nine_field_program_bitwise():
+    """Isothermal ideal MHD with GLM cleaning and a passive scalar: nine
+    fields, 99 manifest nodes (SSP-RK3, the face fluxes of each stage
+    along three axes), stage kernels only; two forwards reproduce the
+    oracle bit for bit for both row counts, and Formura_Forward does too."""
+    from formura_amd.rng import mhd_state
+    lib = hip("mhd_t")
+    assert len(lib.layout.field_names) == 9
+    steps = lib.layout.steps_per_forward
+    st = mhd_state(tuple(lib.layout.intra), 23)
+    want = oracle_run("mhd_t", st, 2)
+    for rows in (16, 4):
+        _say(f"[parity] mhd_t kind=staged rows={rows} stages={lib.num_stages()} steps={2 * steps}")
+        lib.set_kernel(KIND_STAGED, 1, rows)
+        got = device_run(lib, st, 2 * steps)
+        assert_bitwise(got, want, f"mhd_t staged rows={rows}")
+    lib.set_state(st)
+    lib.forward()
+    assert_bitwise(lib.get_state(), oracle_run("mhd_t", st, 1), "mhd_t Formura_Forward")
+
+
 def extra_oracle(name):
     key = name + "/cpu"
     if key not in _cache:
