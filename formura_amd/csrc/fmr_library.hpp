@@ -511,6 +511,7 @@ class StepLibrary {
   };
   std::vector<GraphEntry> graphs;
   bool use_graphs = true;
+  bool prepare_only = false;   // capture what is missing, launch nothing
 
   void drop_graphs() {
     for (auto& g : graphs) FMR_CHECK(hipGraphExecDestroy(g.exec));
@@ -527,7 +528,7 @@ class StepLibrary {
     k = GraphKey{choice.kind, choice.T, choice.by, choice.waves, choice.pf, choice.occ, lx, cur, steps, part};
     for (auto& g : graphs)
       if (g.key == k) {
-        FMR_CHECK(hipGraphLaunch(g.exec, stream));
+        if (!prepare_only) FMR_CHECK(hipGraphLaunch(g.exec, stream));
         cur = g.cur_after;
         return g.launches;
       }
@@ -539,8 +540,20 @@ class StepLibrary {
     FMR_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     FMR_CHECK(hipGraphDestroy(graph));
     graphs.push_back(GraphEntry{k, exec, cur, launches});
-    FMR_CHECK(hipGraphLaunch(exec, stream));
+    if (!prepare_only) FMR_CHECK(hipGraphLaunch(exec, stream));
     return launches;
+  }
+
+  // capture and instantiate the graphs that advance(steps) from the current
+  // buffer will replay, without running them or changing the state: a
+  // caller can keep the one-time capture out of a timed or latency-bound call
+  void prepare(int steps) {
+    if (!use_graphs || steps <= 0) return;
+    const int cur0 = cur;
+    prepare_only = true;
+    advance(steps);
+    prepare_only = false;
+    cur = cur0;
   }
 
   int advance_main_g(int steps) {

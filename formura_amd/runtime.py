@@ -71,6 +71,7 @@ class StepLibrary:
             L.Formura_Num_Stage_Kernels.restype = ctypes.c_int
             L.Formura_Set_Params.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.Formura_Set_Graphs.argtypes = [ctypes.c_int]
+            L.Formura_Prepare_Device.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.argtypes = [ctypes.c_int]
             L.Formura_Device_Field.restype = ctypes.c_void_p
             L.Formura_Reduce_Device.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -159,6 +160,12 @@ class StepLibrary:
         launch eagerly."""
         self.lib.Formura_Set_Graphs(1 if on else 0)
 
+    def prepare(self, steps: int) -> None:
+        """Capture the graphs a following forward_device*(steps) will replay,
+        without running them (no effect with graphs off)."""
+        if self.lib.Formura_Prepare_Device(int(steps)) != 0:
+            raise ValueError("negative step count")
+
     def set_params(self, p: Sequence[float]) -> None:
         arr = (ctypes.c_double * len(p))(*p)
         if self.lib.Formura_Set_Params(arr, len(p)) != 0:
@@ -200,11 +207,13 @@ MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
 
 def march_chunks(occ: int) -> tuple:
     """The work splits (Formura_Set_March_Chunk values) offered for a march
-    variant compiled for `occ` waves per SIMD: runs of 32, 64 or 128 planes,
-    or x whole with one or two (four, for occ >= 4) workgroups per SIMD.  The
-    benchmark's search chooses among exactly these, and the full-size parity
-    tests run every compiled variant with each of them."""
-    return (32, 64, 128, -1, -2) + ((-4,) if occ >= 4 else ())
+    variant compiled for `occ` waves per SIMD: runs of 32, 64, 128 or 256
+    planes (one workgroup per run; the tiles of one run length start
+    together, so neighbouring tiles read their shared ghost rows while they
+    are in L2), or x whole with one or two (four, for occ >= 4) workgroups
+    per SIMD.  The benchmark's search chooses among exactly these, and the
+    full-size parity tests run every compiled variant with each of them."""
+    return (32, 64, 128, 256, -1, -2) + ((-4,) if occ >= 4 else ())
 
 
 def load_program(name: str) -> StepLibrary:

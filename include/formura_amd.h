@@ -79,8 +79,9 @@ int Formura_Num_Stage_Kernels(void);
 /* Current kernel selection. */
 int Formura_Get_Kernel(int *kind, int *T, int *rows, int *waves, int *pf, int *occ, int *lx);
 
-/* Writes up to max_variants (T, rows, waves, pf, occ) tuples of the compiled
- * march variants into out[5*i .. 5*i+4]; returns the number written. */
+/* Writes up to max_variants (kind, T, rows, waves, pf, occ) tuples of the
+ * compiled march variants into out[6*i .. 6*i+5]; returns the number
+ * written. */
 int Formura_List_Kernels(int *out, int max_variants);
 
 /* on != 0 (the default): the launch sequence of a Formura_Forward_Device*
@@ -88,6 +89,13 @@ int Formura_List_Kernels(int *out, int max_variants);
  * starting buffer) and replayed; 0: launched eagerly (and the graphs are
  * released).  Returns 0. */
 int Formura_Set_Graphs(int on);
+
+/* With graphs on: capture and instantiate now the graphs that a following
+ * Formura_Forward_Device*(navi, nsteps) from the current state will replay,
+ * without running them or changing the state, so that the one-time capture
+ * is not part of that call.  Returns 0, or -1 if nsteps < 0.  No counterpart
+ * in the reference (its Formura_Forward launches nothing). */
+int Formura_Prepare_Device(int nsteps);
 
 /* Per-launch scalar parameters (programs compiled with named parameters);
  * returns 0, or -1 if np exceeds 16. */

@@ -149,6 +149,21 @@ def test_graph_replays_match_eager_launches(name, mk):
         lib.forward_device(steps)
         lib.download()
         assert_bitwise(lib.get_state(), eager, f"{name} graph replay {rep}")
+    # Formura_Prepare_Device captures without running: the state stays as it
+    # is, and the run it prepared (a step count not seen before) still gives
+    # the eager bits
+    n2 = steps + 2
+    lib.set_graphs(False)
+    eager2 = device_run(lib, st, n2)
+    lib.set_graphs(True)
+    lib.set_state(st)
+    lib.upload()
+    lib.prepare(n2)
+    lib.download()
+    assert_bitwise(lib.get_state(), st, f"{name} state after prepare")
+    lib.forward_device(n2)
+    lib.download()
+    assert_bitwise(lib.get_state(), eager2, f"{name} prepared run")
 
 
 def test_power_operator_within_stated_bound():
