@@ -294,6 +294,69 @@ def copy_peak():
     return None
 
 
+def _pmc_per_dispatch(outdir, counter, kernel_part):
+    """Mean over dispatches of `counter` (summed over the CSV's rows of one
+    dispatch) for kernels whose name contains `kernel_part`."""
+    import csv
+    per = {}
+    for root, _, files in os.walk(outdir):
+        for fn in files:
+            if fn.endswith("counter_collection.csv"):
+                with open(os.path.join(root, fn), newline="") as f:
+                    for r in csv.DictReader(f):
+                        if kernel_part in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                            per[r["Dispatch_Id"]] = per.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    return sum(per.values()) / len(per) if per else None
+
+
+def measure_traffic(var, lx):
+    """HBM bytes per launch of march variant (var, lx), measured now: four
+    rocprofv3 --pmc runs as child processes (one counter each, as rocprofv3
+    collects one pass per run): FETCH_SIZE and WRITE_SIZE of 12 steps of this
+    bench with the variant fixed, and of the 1 GiB copy kernel of
+    formura_amd/csrc/hbm_copy.hip, whose known bytes calibrate the counters'
+    units.  Returns (bytes, details) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    from formura_amd import build
+    prof = shutil.which("rocprofv3")
+    exe = os.path.join(build.BUILD, "hbm_copy")
+    if not prof or not os.path.exists(exe):
+        return None, "rocprofv3 or hbm_copy missing"
+    tag = ",".join(str(x) for x in list(var) + [lx])
+    bench = [sys.executable, os.path.abspath(__file__), "--variant", tag, "--steps", "12", "--warmup", "0",
+             "--no-extra", "--no-cpu-baseline", "--no-traffic"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    vals = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for key, counter, cmd, kpart in (("fetch", "FETCH_SIZE", bench, "march"), ("write", "WRITE_SIZE", bench, "march"),
+                                         ("copy_fetch", "FETCH_SIZE", [exe], "copy"),
+                                         ("copy_write", "WRITE_SIZE", [exe], "copy")):
+            d = os.path.join(tmp, key)
+            log(f"traffic: rocprofv3 --pmc {counter} ({key}) ...")
+            try:
+                r = subprocess.run(["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
+                                    "-d", d, "-o", key, "--"] + cmd, cwd="/tmp", env=env,
+                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=150)
+            except (subprocess.TimeoutExpired, OSError) as e:
+                return None, f"{key}: {e}"
+            v = _pmc_per_dispatch(d, counter, kpart)
+            if r.returncode != 0 and v is None:
+                return None, f"{key}: rocprofv3 exit {r.returncode}"
+            if v is None or v <= 0:
+                return None, f"{key}: no {counter} rows"
+            vals[key] = v
+    copy_bytes = 1 << 30
+    rf = copy_bytes / (vals["copy_fetch"] * 1024)      # bytes per FETCH_SIZE KiB unit
+    wf = copy_bytes / (vals["copy_write"] * 1024)
+    read_b, write_b = vals["fetch"] * 1024 * rf, vals["write"] * 1024 * wf
+    return int(read_b + write_b), {"FETCH_SIZE_KiB": round(vals["fetch"], 1), "WRITE_SIZE_KiB": round(vals["write"], 1),
+                                   "copy_FETCH_SIZE_KiB": round(vals["copy_fetch"], 1),
+                                   "copy_WRITE_SIZE_KiB": round(vals["copy_write"], 1),
+                                   "read_bytes": int(read_b), "write_bytes": int(write_b)}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo", encoding="utf-8") as f:
@@ -312,6 +375,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
     ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,occ,lx: skip the search")
     ap.add_argument("--only", default="", help="run only this line of 'extra' (e.g. for a profile)")
     a = ap.parse_args()
@@ -357,17 +421,24 @@ def main():
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     s0, _ = lib.reduce(0)
 
-    # HBM bytes per launch of this kernel variant, from rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs scaled by the copy-kernel calibration (profiles/); null
-    # for a variant that was not profiled
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_heat512.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            t = json.load(f)
-        rec = t.get("variants", {}).get(",".join(str(x) for x in list(var) + [lx]))
-        if rec:
-            traffic = rec["hbm_bytes_per_launch"]
+    # HBM bytes per launch of this kernel variant: measured in this run by
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (child processes, after the
+    # timed region) scaled by the copy-kernel calibration; if that cannot run,
+    # the committed measurement of the same variant (profiles/), else null
+    traffic, traffic_source, traffic_detail = None, None, None
+    if not a.no_traffic and not a.variant:
+        traffic, traffic_detail = measure_traffic(var, lx)
+        traffic_source = "rocprofv3 --pmc in this run" if traffic else None
+        log(f"traffic: {traffic} {traffic_detail}")
+    if traffic is None:
+        tfile = os.path.join(ROOT, "profiles", "traffic_heat512.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                t = json.load(f)
+            rec = t.get("variants", {}).get(",".join(str(x) for x in list(var) + [lx]))
+            if rec:
+                traffic = rec["hbm_bytes_per_launch"]
+                traffic_source = "profiles/traffic_heat512.json (an earlier session)"
 
     out = {
         "metric": "GLUP/s & %HBM roofline; batched/surrogate speedup at <=1e-3 rel L2 vs FP64",
@@ -388,6 +459,12 @@ def main():
                                                        "march_chunk": lx, "march_chunk_note": "lx > 0: planes per workgroup run; lx < 0: workgroups per SIMD"}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_vs_algorithmic": round(traffic / bytes_per_launch, 3) if traffic else None,
+                     # the bytes HBM actually moved per second (traffic / launch time) and their
+                     # fraction of the 8 TB/s peak; `achieved` and `frac` count algorithmic bytes
+                     "moved_GBs": round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "moved_frac": round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                     "traffic_source": traffic_source, "traffic_counters": traffic_detail if traffic else None,
                      "measured_copy_peak": peak_copy,
                      "frac_of_measured_copy_peak": round(achieved / peak_copy, 4) if peak_copy else None},
         "t_sweep_glups": sweep,

@@ -54,7 +54,7 @@ def test_forward_and_gradients_match_oracle():
     inc_ref = ref["out"] - state
     e_inc = np.linalg.norm(inc_gpu - inc_ref) / np.linalg.norm(inc_ref)
     _say(f"[surrogate] forward increment rel L2 error {e_inc:.3e}")
-    assert e_inc < 2e-2
+    assert e_inc < 2e-3           # measured 3.8e-4: BF16 rounding ties and FP32 sum order
     assert np.linalg.norm(out - ref["out"]) / np.linalg.norm(ref["out"]) < 1e-5
     _say("[surrogate] gradients")
     loss = sur.gradients(bs, bf, bt, B)
@@ -69,7 +69,7 @@ def test_forward_and_gradients_match_oracle():
         for name, sl in (("W", slice(off, off + nw)), ("b", slice(off + nw, off + nw + nb))):
             e = np.linalg.norm(g[sl] - gref[sl]) / max(np.linalg.norm(gref[sl]), 1e-30)
             _say(f"[surrogate] layer {l + 1} d{name}: rel L2 error {e:.3e}")
-            assert e < 2e-2, f"layer {l + 1} d{name}"
+            assert e < 5e-4, f"layer {l + 1} d{name}"     # measured 8e-6 .. 4.3e-5
         off += nw + nb
     for b in (bs, bt, bf, bo):
         b.free()
