@@ -255,6 +255,7 @@ class TrainResult:
     seconds: float = 0.0
     steps: int = 0
     scale: tuple = (1.0, 1.0)
+    kept: int = 0                  # trajectories kept in HBM for replay
 
 
 def forward_flops(H: int = 256, W: int = 256) -> float:
@@ -270,7 +271,7 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 
 
 def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
-          seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 96.0, unroll: int = 0,
+          seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 128.0, unroll: int = 0,
           log=print) -> TrainResult:
     """Train on trajectories of the FP32 stencil: generate, then replay.
 
@@ -302,6 +303,7 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     if n_keep < 1:
         raise ValueError("store_gib too small for one trajectory")
     res = TrainResult()
+    res.kept = n_keep
     t0 = time.perf_counter()
     # generation: snaps[r] holds the n_keep kept states after super-step r
     snaps = sur.buffer(nb * n_keep * (rounds + 1))
@@ -673,14 +675,16 @@ def main(argv=None):
     ap.add_argument("--passes", type=int, default=1)
     ap.add_argument("--noise", type=float, default=0.0)
     ap.add_argument("--unroll", type=int, default=0, help="pushforward: up to this many network passes before the target")
+    ap.add_argument("--store-gib", type=float, default=128.0,
+                    help="HBM for the kept trajectories (2000 x 127 states of 256^2 x 2 FP32: 124 GiB)")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
     a = ap.parse_args(argv)
     if a.command == "train":
         res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, passes=a.passes, noise=a.noise,
-                    unroll=a.unroll, log=lambda m: print(m, flush=True))
+                    store_gib=a.store_gib, unroll=a.unroll, log=lambda m: print(m, flush=True))
         theta = Surrogate.get(a.batch).get_params()
-        meta = {"trajectories": a.trajectories, "traj_steps": a.traj_steps, "adam_steps": res.steps,
+        meta = {"trajectories": a.trajectories, "kept": res.kept, "traj_steps": a.traj_steps, "adam_steps": res.steps,
                 "passes": a.passes, "noise": a.noise, "unroll": a.unroll, "lr": a.lr, "batch": a.batch,
                 "seconds": round(res.seconds, 1), "final_loss": float(np.mean(res.losses[-100:]))}
         save_weights(a.out, theta, res.scale, meta)
