@@ -162,10 +162,11 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
     wave to hide latency) in workgroups of 8 waves (a CU): one and two z-sites
     per lane (two only when nz is even), as many rows per wave as the
     register model allows (one site per lane: two rows fewer, as slack for
-    the scheduler), with one site per lane also one row fewer than that,
-    prefetching one or two planes (on heat 512^3 FP64 at T = 4, 6 rows ran
+    the scheduler), also one row fewer than that (one site per lane:
+    prefetching one or two planes; on heat 512^3 FP64 at T = 4, 6 rows ran
     5 % faster than 7, and at T = 3 8 rows 12 % faster than 9, in one
-    session), and the tile whose stored rows are the largest divisor of ny
+    session; two sites per lane at T = 3, 4 rows ran 948 GLUP/s against 923
+    for 5), and the tile whose stored rows are the largest divisor of ny
     that fits."""
     words = 2 if real == "double" else 1
     depths = range(3, 5) if real == "double" else range(3, 7)
@@ -181,6 +182,8 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
                 out.append((kind, T, by, W, 1, 2))
                 if vz == 1 and W * (by - 1) - 2 * T >= 4 * T:
                     out += [(kind, T, by - 1, W, 1, 2), (kind, T, by - 1, W, 2, 2)]
+                elif vz == 2 and W * (by - 1) - 2 * T >= 4 * T:
+                    out.append((kind, T, by - 1, W, 1, 2))
             if ny and by8 >= 1 and marchv_lds_bytes(T, 8, nf, words, vz) <= MAX_MARCHV_LDS:
                 fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
                 if fits and (kind, T, max(fits), 8, 1, 2) not in out:

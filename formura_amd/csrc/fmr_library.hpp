@@ -154,6 +154,26 @@ constexpr int march_stored_z(int kind, int T) {
                             : (kind == KIND_MARCHV2 ? 2 * WAVE - 2 * ((T + 1) / 2 * 2) : WAVE - 2 * T);
 }
 
+// The z tiling of a march launch over nz sites: stored sites per tile row and
+// ghost sites per side.  For two z-sites per lane the stored width is cut
+// down to whole 128-byte lines when that takes no more tiles (nz = 512 in
+// FP64: 112 instead of 120, five tiles either way): every tile then stores
+// whole lines, none shares a line with its neighbour, and the wider ghost
+// is lanes the tile computes anyway.
+inline void march_tile_z(int kind, int T, int nz, int elem, int& stored, int& ghost) {
+  const int zw = kind == KIND_MARCHV2 ? 2 * WAVE : WAVE;
+  stored = march_stored_z(kind, T);
+  ghost = (zw - stored) / 2;
+  if (kind == KIND_MARCHV2) {
+    const int line = 128 / elem;
+    const int s1 = stored / line * line;
+    if (s1 > 0 && (nz + s1 - 1) / s1 == (nz + stored - 1) / stored) {
+      stored = s1;
+      ghost = (zw - s1) / 2;
+    }
+  }
+}
+
 struct Variant {
   int kind;
   int T;       // time levels fused per launch
@@ -165,11 +185,11 @@ struct Variant {
 
 template <class P>
 using MarchLauncher = void (*)(Fields<typename P::real>, Fields<typename P::real>, int, int, int, int, int, int,
-                               int, Params, hipStream_t);
+                               int, int, Params, hipStream_t);
 
 template <class P, int T, int BY, int W, int PF, int OCC>
 void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
-                    int nty, int ntz, int chunk, Params prm, hipStream_t s) {
+                    int nty, int ntz, int chunk, int /*ghost: T per side*/, Params prm, hipStream_t s) {
   if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
     fprintf(stderr, "formura_amd: march kernel needs a plane (NY*NZ sites) smaller than 2 GiB\n");
     abort();
@@ -180,7 +200,7 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
 
 template <class P, int T, int BY, int W, int VZ, int PF, int OCC>
 void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
-                   int nty, int ntz, int chunk, Params prm, hipStream_t s) {
+                   int nty, int ntz, int chunk, int ghost, Params prm, hipStream_t s) {
   if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
     fprintf(stderr, "formura_amd: march kernel needs a plane (NY*NZ sites) smaller than 2 GiB\n");
     abort();
@@ -189,11 +209,15 @@ void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, in
     fprintf(stderr, "formura_amd: march kernel with %d z-sites per lane needs NZ divisible by %d\n", VZ, VZ);
     abort();
   }
+  if (ghost < (T + VZ - 1) / VZ * VZ || ghost % VZ != 0 || 2 * ghost >= WAVE * VZ) {
+    fprintf(stderr, "formura_amd: march kernel: bad ghost width %d\n", ghost);
+    abort();
+  }
   // the x-carry with one z-site per lane (P::XC); with two, the window keeps
   // the plane behind (fewer registers at the peak of a row's evaluation)
   constexpr bool XC = VZ == 1 && P::XC;
   hipLaunchKernelGGL((marchv_kernel<P, T, BY, W, VZ, XC, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out,
-                     nx, ny, nz, nty, ntz, chunk, prm);
+                     nx, ny, nz, nty, ntz, chunk, ghost, prm);
 }
 
 // one manifest node of a multi-stage step (stage_kernel<S>)
@@ -414,15 +438,25 @@ class StepLibrary {
   }
 
   // work split of a march launch (see march3d_kernel): lx > 0 cuts x into
-  // chunks of lx planes, one workgroup per (chunk, tile); lx < 0 keeps x whole
-  // and launches -lx workgroups per SIMD of the device, balanced to a plane.
-  // The workgroup count is a multiple of 8 (one share per XCD).
+  // chunks of lx planes, one workgroup per (chunk, tile); -100 > lx launches
+  // (-lx - 100) waves per SIMD of the device and cuts x into the chunks that
+  // give each workgroup one run of its own (ntile * nx / workgroups planes;
+  // with as many workgroups as are resident at once, every tile of a chunk
+  // starts together and no second round of workgroups is left part-full);
+  // other lx < 0 keep x whole and launch -lx waves per SIMD, balanced to a
+  // plane.  The workgroup count is a multiple of 8 (one share per XCD).
   void march_split(int nty, int ntz, int W, int& groups, int& chunk) const {
     const long long ntile = (long long)nty * ntz;
     long long g;
     if (lx > 0) {
       chunk = lx < n[0] ? lx : n[0];
       g = ntile * ((n[0] + chunk - 1) / chunk);
+    } else if (lx < -100) {
+      g = (long long)(-lx - 100) * simds / W;
+      g = (g + N_XCD - 1) / N_XCD * N_XCD;
+      if (g > ntile * n[0]) g = ntile * n[0];
+      const long long c = (ntile * n[0] + g - 1) / g;
+      chunk = (int)(c < 1 ? 1 : (c > n[0] ? n[0] : c));
     } else {
       chunk = n[0];
       g = (long long)(-lx) * simds / W;
@@ -435,11 +469,12 @@ class StepLibrary {
   void launch_march(const VariantEntry<P>* e) {
     const int T = e->v.T, rows = e->v.by * e->v.waves;
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
-    const int zw = march_stored_z(e->v.kind, T);
+    int zw = 0, ghost = 0;
+    march_tile_z(e->v.kind, T, n[2], (int)sizeof(real), zw, ghost);
     const int ntz = (n[2] + zw - 1) / zw;
     int groups = 0, chunk = 0;
     march_split(nty, ntz, e->v.waves, groups, chunk);
-    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], groups, nty, ntz, chunk, prm, stream);
+    e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], groups, nty, ntz, chunk, ghost, prm, stream);
     FMR_CHECK(hipGetLastError());
     cur = 1 - cur;
   }

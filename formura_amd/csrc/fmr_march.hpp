@@ -37,6 +37,10 @@
 
 namespace fmr {
 
+// cache policy of the march's state stores: non-temporal (streamed past the
+// caches; plain stores measured the same)
+constexpr int MV_STORE = BUF_NT;
+
 template <class real, int VZ>
 struct VecIO;
 
@@ -47,7 +51,8 @@ struct VecIO<double, 1> {
   }
   __device__ __forceinline__ static void store(const double (&v)[1], __amdgpu_buffer_rsrc_t r, unsigned off,
                                                unsigned soff) {
-    buf_store_nt(v[0], r, off, soff);
+    typedef int V2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v[0]), r, off, soff, MV_STORE);
   }
 };
 
@@ -65,7 +70,7 @@ struct VecIO<double, 2> {
     D2 d;
     d.x = v[0];
     d.y = v[1];
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, d), r, off, soff, BUF_NT);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, d), r, off, soff, MV_STORE);
   }
 };
 
@@ -76,7 +81,7 @@ struct VecIO<float, 1> {
   }
   __device__ __forceinline__ static void store(const float (&v)[1], __amdgpu_buffer_rsrc_t r, unsigned off,
                                                unsigned soff) {
-    buf_store_nt(v[0], r, off, soff);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[0]), r, off, soff, MV_STORE);
   }
 };
 
@@ -94,7 +99,7 @@ struct VecIO<float, 2> {
     F2 d;
     d.x = v[0];
     d.y = v[1];
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, d), r, off, soff, BUF_NT);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, d), r, off, soff, MV_STORE);
   }
 };
 
@@ -105,7 +110,7 @@ struct MarchV {
   static constexpr int NC = XC ? P::NC : 0;     // carried values per site
   static constexpr int NP = XC ? 2 : 3;         // window planes per level
   static constexpr int ROWS = BY * W;
-  static constexpr int G = (T + VZ - 1) / VZ * VZ;   // ghost z-sites per side (whole lanes)
+  static constexpr int G = (T + VZ - 1) / VZ * VZ;   // fewest ghost z-sites per side (whole lanes)
   static constexpr int ZW = WAVE * VZ;               // z-sites per row of a tile
   static_assert(T >= 1 && ROWS > 2 * T, "tile must be taller than its ghost rows");
   static_assert(ZW > 2 * G, "tile must be wider than its ghost lanes");
@@ -121,6 +126,8 @@ struct MarchV {
   int wave, lane, half;
   int wave_above, wave_below;   // (wave - 1) and (wave + 1) modulo W
   int nx, ny, nz;
+  int gz;              // ghost z-sites per side: G, or more (a multiple of VZ) so that the stored
+                       // range of every tile starts on a 128-byte line
   int zg;              // wrapped z of this lane's first site
   unsigned zoff;       // byte offset of this lane's first site within a row
   unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane or a lane past the grid end
@@ -314,7 +321,7 @@ struct MarchV {
     const int tz = tile % ntz;
     const int ty = tile / ntz;
     const int y0 = ty * (ROWS - 2 * T) - T;                // unwrapped y of tile row 0
-    const int z0 = tz * (ZW - 2 * G) - G;                  // unwrapped z of lane 0's first site
+    const int z0 = tz * (ZW - 2 * gz) - gz;                // unwrapped z of lane 0's first site
     const int zu = z0 + lane * VZ;                         // unwrapped z of this lane's first site
     zg = wrap(zu, nz);
     const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
@@ -322,7 +329,7 @@ struct MarchV {
     for (int r = 0; r < BY; ++r)
       roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + wave * BY + r, ny) * nz * (int)sizeof(real));
     zoff = (unsigned)(zg * sizeof(real));
-    zst = (lane * VZ >= G && lane * VZ < ZW - G && zu < nz) ? zoff : BUF_OOB;
+    zst = (lane * VZ >= gz && lane * VZ < ZW - gz && zu < nz) ? zoff : BUF_OOB;
     int xl = wrap(x0 - T, nx);
     load_plane<0>(in, xl, plane_bytes);
     if constexpr (PF > 1) {
@@ -342,7 +349,7 @@ struct MarchV {
 template <class P, int T, int BY, int W, int VZ, bool XC, int PF, int OCC>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
 marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int nty, int ntz,
-              int L, Params prm) {
+              int L, int gz, Params prm) {
   using M = MarchV<P, T, BY, W, VZ, XC, PF>;
   const int G = gridDim.x;
   const int b = blockIdx.x;
@@ -362,6 +369,7 @@ marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx,
   m.nx = nx;
   m.ny = ny;
   m.nz = nz;
+  m.gz = gz;
   if constexpr (W > 1) {
     __shared__ typename M::Edge edge_buf;
     m.edge = &edge_buf;

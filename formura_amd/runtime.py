@@ -205,15 +205,23 @@ KIND_MARCHV2 = 4     # march with the x-carry, two z-sites per lane (marchv_kern
 MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
 
 
+def balanced_runs(occ: int) -> int:
+    """The Formura_Set_March_Chunk value of the balanced-run split for a
+    variant compiled for `occ` waves per SIMD: as many workgroups as are
+    resident at once, x cut so that each has one run of its own."""
+    return -100 - occ
+
+
 def march_chunks(occ: int) -> tuple:
     """The work splits (Formura_Set_March_Chunk values) offered for a march
     variant compiled for `occ` waves per SIMD: runs of 32, 64, 128 or 256
     planes (one workgroup per run; the tiles of one run length start
     together, so neighbouring tiles read their shared ghost rows while they
-    are in L2), or x whole with one or two (four, for occ >= 4) workgroups
-    per SIMD.  The benchmark's search chooses among exactly these, and the
-    full-size parity tests run every compiled variant with each of them."""
-    return (32, 64, 128, 256, -1, -2) + ((-4,) if occ >= 4 else ())
+    are in L2), balanced runs (one round of resident workgroups, one run
+    each), or x whole with one or two (four, for occ >= 4) waves per SIMD.
+    The benchmark's search chooses among exactly these, and the full-size
+    parity tests run every compiled variant with each of them."""
+    return (32, 64, 128, 256, balanced_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
 
 
 def load_program(name: str) -> StepLibrary:

@@ -63,8 +63,10 @@ int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
 /* Work split of the march kernel.  lx > 0: x is cut into chunks of lx
- * planes, one workgroup per (chunk, tile); lx < 0: x is kept whole and -lx
- * workgroups per SIMD of the device share the (tile, plane) items evenly.
+ * planes, one workgroup per (chunk, tile); lx < -100: (-lx - 100) waves per
+ * SIMD of the device, x cut into the chunks that give each workgroup one run
+ * of its own (balanced runs); other lx < 0: x is kept whole and -lx waves
+ * per SIMD of the device share the (tile, plane) items evenly.
  * Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
 
