@@ -66,7 +66,7 @@ PROGRAMS: Dict[str, tuple] = {
     "fveuler_t": (_ex("fv-euler-3d.fmr"), _ex("fv-euler-3d.yaml"), "faithful", (),
                   {"intra_node_shape": [22, 26, 34], "monitor_interval": 2}, None),
     # isothermal ideal MHD with GLM divergence cleaning and a passive scalar:
-    # nine fields, 99 manifest nodes (SSP-RK3 stages), stage kernels only
+    # nine fields, 108 manifest nodes (SSP-RK3 stages), stage kernels only
     "mhd": (_ex("mhd-glm-3d.fmr"), _ex("mhd-glm-3d.yaml"), "faithful", (), None, None),
     "mhd_t": (_ex("mhd-glm-3d.fmr"), _ex("mhd-glm-3d.yaml"), "faithful", (),
               {"intra_node_shape": [20, 24, 28], "monitor_interval": 2}, None),

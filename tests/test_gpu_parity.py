@@ -307,7 +307,7 @@ def test_large_multistage_program_bitwise():
 
 def test_mhd_nine_field_program_bitwise():
     """Isothermal ideal MHD with GLM cleaning and a passive scalar: nine
-    fields, 99 manifest nodes (SSP-RK3, the face fluxes of each stage
+    fields, 108 manifest nodes (SSP-RK3, the face fluxes of each stage
     along three axes), stage kernels only; two forwards reproduce the
     oracle bit for bit for both row counts, and Formura_Forward does too."""
     from formura_amd.rng import mhd_state

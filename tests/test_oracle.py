@@ -429,7 +429,7 @@ def mhd_oracle(tmp_path_factory):
 
 
 def test_mhd_oracle_matches_numpy_restatement(mhd_oracle):
-    """99 manifest nodes (three SSP-RK3 stages, each with the face fluxes of
+    """108 manifest nodes (three SSP-RK3 stages, each with the face fluxes of
     nine fields along three axes, and two intermediate states) in two
     forwards of two steps: the oracle's loop nests against the numpy
     restatement, bit for bit.  The restatement without the Maxwell stress,
