@@ -438,7 +438,10 @@ class StepLibrary {
   }
 
   // work split of a march launch (see march3d_kernel): lx > 0 cuts x into
-  // chunks of lx planes, one workgroup per (chunk, tile); -100 > lx launches
+  // chunks of lx planes, one workgroup per (chunk, tile); -200 > lx cuts x
+  // into the most chunks k whose k * ntile workgroups are resident at once
+  // with (-lx - 200) waves per SIMD (at least one), one workgroup per (chunk,
+  // tile): every workgroup one run, all of them in step; -100 > lx launches
   // (-lx - 100) waves per SIMD of the device and cuts x into the chunks that
   // give each workgroup one run of its own (ntile * nx / workgroups planes;
   // with as many workgroups as are resident at once, every tile of a chunk
@@ -450,6 +453,13 @@ class StepLibrary {
     long long g;
     if (lx > 0) {
       chunk = lx < n[0] ? lx : n[0];
+      g = ntile * ((n[0] + chunk - 1) / chunk);
+    } else if (lx < -200) {
+      const long long res = (long long)(-lx - 200) * simds / W;
+      long long k = res / ntile;
+      if (k < 1) k = 1;
+      if (k > n[0]) k = n[0];
+      chunk = (int)((n[0] + k - 1) / k);
       g = ntile * ((n[0] + chunk - 1) / chunk);
     } else if (lx < -100) {
       g = (long long)(-lx - 100) * simds / W;

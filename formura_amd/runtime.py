@@ -212,16 +212,25 @@ def balanced_runs(occ: int) -> int:
     return -100 - occ
 
 
+def whole_runs(occ: int) -> int:
+    """The Formura_Set_March_Chunk value of the whole-run split for a variant
+    compiled for `occ` waves per SIMD: x cut into the most chunks k per tile
+    whose k x tiles workgroups are resident at once, one run each."""
+    return -200 - occ
+
+
 def march_chunks(occ: int) -> tuple:
     """The work splits (Formura_Set_March_Chunk values) offered for a march
     variant compiled for `occ` waves per SIMD: runs of 32, 64, 128 or 256
     planes (one workgroup per run; the tiles of one run length start
     together, so neighbouring tiles read their shared ghost rows while they
     are in L2), balanced runs (one round of resident workgroups, one run
-    each), or x whole with one or two (four, for occ >= 4) waves per SIMD.
+    each), whole runs (k runs per tile, k x tiles workgroups resident at
+    once, all in step), or x whole with one or two (four, for occ >= 4)
+    waves per SIMD.
     The benchmark's search chooses among exactly these, and the full-size
     parity tests run every compiled variant with each of them."""
-    return (32, 64, 128, 256, balanced_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
+    return (32, 64, 128, 256, balanced_runs(occ), whole_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
 
 
 def load_program(name: str) -> StepLibrary:

@@ -58,16 +58,20 @@ int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
  * workgroup, 4 (also for 0) or 16; other arguments ignored),
  * kind 1 = temporally blocked march with T fused steps, `rows` y-rows per lane,
  * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
- * ahead, compiled for at least `occ` waves per SIMD.  Returns 0, or -1 if that
- * variant is not compiled into the library. */
+ * ahead, compiled for at least `occ` waves per SIMD; kinds 3 and 4 = the same
+ * march with the x-difference carried from plane to plane and one z-site per
+ * lane (3), or two consecutive z-sites per lane (4).  Returns 0, or -1 if
+ * that variant is not compiled into the library. */
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
 /* Work split of the march kernel.  lx > 0: x is cut into chunks of lx
- * planes, one workgroup per (chunk, tile); lx < -100: (-lx - 100) waves per
- * SIMD of the device, x cut into the chunks that give each workgroup one run
- * of its own (balanced runs); other lx < 0: x is kept whole and -lx waves
- * per SIMD of the device share the (tile, plane) items evenly.
- * Returns 0, or -1 for 0. */
+ * planes, one workgroup per (chunk, tile); lx < -200: x is cut into the
+ * most chunks k per tile whose k x tiles workgroups are resident at once with
+ * (-lx - 200) waves per SIMD, one workgroup per (chunk, tile) (whole runs);
+ * -200 <= lx < -100: (-lx - 100) waves per SIMD of the device, x cut into the
+ * chunks that give each workgroup one run of its own (balanced runs); other
+ * lx < 0: x is kept whole and -lx waves per SIMD of the device share the
+ * (tile, plane) items evenly.  Returns 0, or -1 for 0. */
 int Formura_Set_March_Chunk(int lx);
 
 /* Number of manifest nodes the stage kernels (kind 2) evaluate per step; 0
