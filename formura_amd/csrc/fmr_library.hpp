@@ -447,7 +447,9 @@ class StepLibrary {
   // with as many workgroups as are resident at once, every tile of a chunk
   // starts together and no second round of workgroups is left part-full);
   // other lx < 0 keep x whole and launch -lx waves per SIMD, balanced to a
-  // plane.  The workgroup count is a multiple of 8 (one share per XCD).
+  // plane.  The kernels spread any workgroup count over the XCDs in
+  // contiguous ranges (xcd_order), so the count is exactly the number of
+  // runs wherever the split is one run per workgroup.
   void march_split(int nty, int ntz, int W, int& groups, int& chunk) const {
     const long long ntile = (long long)nty * ntz;
     long long g;
@@ -463,7 +465,6 @@ class StepLibrary {
       g = ntile * ((n[0] + chunk - 1) / chunk);
     } else if (lx < -100) {
       g = (long long)(-lx - 100) * simds / W;
-      g = (g + N_XCD - 1) / N_XCD * N_XCD;
       if (g > ntile * n[0]) g = ntile * n[0];
       const long long c = (ntile * n[0] + g - 1) / g;
       chunk = (int)(c < 1 ? 1 : (c > n[0] ? n[0] : c));
@@ -472,8 +473,7 @@ class StepLibrary {
       g = (long long)(-lx) * simds / W;
       if (g > ntile * n[0]) g = ntile * n[0];
     }
-    g = (g + N_XCD - 1) / N_XCD * N_XCD;
-    groups = (int)(g < N_XCD ? N_XCD : g);
+    groups = (int)(g < 1 ? 1 : g);
   }
 
   void launch_march(const VariantEntry<P>* e) {

@@ -353,7 +353,7 @@ marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx,
   using M = MarchV<P, T, BY, W, VZ, XC, PF>;
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int g = (b % N_XCD) * (G / N_XCD) + b / N_XCD;
+  const int g = xcd_order(b, G);
   const long long ntile = (long long)nty * ntz;
   const long long total = ntile * nx;
   const int nfull = nx / L;

@@ -53,6 +53,15 @@ constexpr int MAX_FIELDS = 16;
 constexpr int MAX_PARAMS = 16;
 constexpr int N_XCD = 8;   // MI355X: 8 accelerator complex dies, each with its own L2
 
+// Workgroup b of G is dispatched to XCD b % N_XCD.  xcd_order numbers the
+// workgroups so that each XCD holds one contiguous range of [0, G) (ranges of
+// G / N_XCD or one more, the longer ones first): consecutive work items share
+// an XCD and its L2.  A bijection of [0, G) for every G >= 1.
+__device__ __forceinline__ int xcd_order(int b, int G) {
+  const int x = b % N_XCD, q = G / N_XCD, rem = G % N_XCD;
+  return x * q + (x < rem ? x : rem) + b / N_XCD;
+}
+
 template <class real>
 struct Fields {
   real* f[MAX_FIELDS];
@@ -613,7 +622,7 @@ march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx
   using M = March3D<P, T, BY, W, PF>;
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int g = (b % N_XCD) * (G / N_XCD) + b / N_XCD;   // neighbouring runs on one XCD (shared L2)
+  const int g = xcd_order(b, G);   // neighbouring runs on one XCD (shared L2)
   const long long ntile = (long long)nty * ntz;
   const long long total = ntile * nx;
   const int nfull = nx / L;                              // chunks of exactly L planes
