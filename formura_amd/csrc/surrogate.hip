@@ -473,30 +473,66 @@ __global__ void __launch_bounds__(256) wgrad_mfma(const bf16* __restrict__ a, co
     }
 }
 
-// bias gradient partials: part[chunk][n] = sum over the chunk's pixels of dz[p][n]
+// bias gradient partials: part[chunk][n] = sum over the chunk's pixels of
+// dz[p][n].  256 threads per chunk, one pixel per thread at a time: thread t
+// sums pixels t, t + 256, ... of the chunk in that order (each pixel's CS
+// channels are one run of 16-byte loads), then thread n < CO adds the 256
+// threads' sums of channel n in thread order.  Fixed order, so the same bits
+// on every run.
 template <int CO>
-__global__ void __launch_bounds__(64) bgrad(const bf16* __restrict__ dz, float* __restrict__ part, int B, int H,
-                                            int W, int RC) {
-  constexpr int CS = CO < 8 ? 8 : CO;
+__global__ void __launch_bounds__(256) bgrad(const bf16* __restrict__ dz, float* __restrict__ part, int B, int H,
+                                             int W, int RC) {
+  constexpr int CS = CO < 8 ? 8 : CO;   // channel stride of dz (BF16 elements): 16-byte multiples
+  static_assert(CS % 8 == 0, "whole 16-byte loads per pixel");
+  __shared__ float sh[256][CO + 1];
   const int chunk = blockIdx.x;
   const int b = chunk / (H / RC);
   const int r0 = (chunk % (H / RC)) * RC;
-  for (int n = threadIdx.x; n < CO; n += 64) {
+  const int P = RC * W;
+  const bf16* base = dz + ((size_t)b * H + r0) * W * CS;
+  float acc[CO];
+#pragma unroll
+  for (int n = 0; n < CO; ++n) acc[n] = 0.f;
+  for (int p = threadIdx.x; p < P; p += 256) {
+    const uint4* px = reinterpret_cast<const uint4*>(base + (size_t)p * CS);
+#pragma unroll
+    for (int v = 0; v < CS / 8; ++v) {
+      const uint4 q = px[v];
+      const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = 8 * v + j;
+        if (n < CO) acc[n] += bf2f((bf16)((w4[j / 2] >> (16 * (j % 2))) & 0xffffu));
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < CO; ++n) sh[threadIdx.x][n] = acc[n];
+  __syncthreads();
+  if (threadIdx.x < CO) {
     float s = 0.f;
-    for (int y = r0; y < r0 + RC; ++y)
-      for (int x = 0; x < W; ++x) s += bf2f(dz[(((size_t)b * H + y) * W + x) * CS + n]);
-    part[(size_t)chunk * CO + n] = s;
+    for (int t = 0; t < 256; ++t) s += sh[t][threadIdx.x];
+    part[(size_t)chunk * CO + threadIdx.x] = s;
   }
 }
 
 // sum partials over chunks in a fixed order: out[i] = sum_c part[c * n + i]
-__global__ void sum_chunks(const float* __restrict__ part, float* __restrict__ out, int chunks, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += part[(size_t)c * n + i];
-    out[i] = s;
-  }
+// as four quarter sums over consecutive chunk ranges, each in chunk order,
+// added in quarter order (64 outputs x 4 quarters per 256-thread workgroup)
+__global__ void __launch_bounds__(256) sum_chunks(const float* __restrict__ part, float* __restrict__ out, int chunks,
+                                                  int n) {
+  __shared__ float sh[4][64];
+  const int t = threadIdx.x & 63, k = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + t;
+  const int c0 = (int)((long long)chunks * k / 4), c1 = (int)((long long)chunks * (k + 1) / 4);
+  float s = 0.f;
+  if (i < n)
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * n + i];
+  sh[k][t] = s;
+  __syncthreads();
+  if (k == 0 && i < n) out[i] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
 }
+inline int sum_chunks_blocks(size_t n) { return (int)((n + 63) / 64); }
 
 // x += sigma * N(0,1) with a counter-based hash (Box-Muller over two uniforms):
 // input-noise augmentation for training
@@ -666,7 +702,7 @@ double backward(const float* target, int B) {
     // sum the chunks, then fold the hi and lo halves (tied weights)
     const int ncol_pad = 9 * ci;
     float* sum = net.part + (size_t)chunks * COUT[l] * ncol_pad;
-    hipLaunchKernelGGL(sum_chunks, dim3(grid_for((size_t)COUT[l] * ncol_pad, 256)), dim3(256), 0, net.stream,
+    hipLaunchKernelGGL(sum_chunks, dim3(sum_chunks_blocks((size_t)COUT[l] * ncol_pad)), dim3(256), 0, net.stream,
                        net.part, sum, chunks, COUT[l] * ncol_pad);
     if (ci == CIN[l]) {
       SCHECK(hipMemcpyAsync(net.g + net.off_w[l], sum, sizeof(float) * COUT[l] * ncol, hipMemcpyDeviceToDevice,
@@ -677,10 +713,11 @@ double backward(const float* target, int B) {
     }
     // bias gradient
     if (COUT[l] == 2)
-      hipLaunchKernelGGL((bgrad<2>), dim3(chunks), dim3(64), 0, net.stream, dz, net.part, B, H, W, net.RC);
+      hipLaunchKernelGGL((bgrad<2>), dim3(chunks), dim3(256), 0, net.stream, dz, net.part, B, H, W, net.RC);
     else
-      hipLaunchKernelGGL((bgrad<48>), dim3(chunks), dim3(64), 0, net.stream, dz, net.part, B, H, W, net.RC);
-    hipLaunchKernelGGL(sum_chunks, dim3(1), dim3(64), 0, net.stream, net.part, net.g + net.off_b[l], chunks,
+      hipLaunchKernelGGL((bgrad<48>), dim3(chunks), dim3(256), 0, net.stream, dz, net.part, B, H, W, net.RC);
+    hipLaunchKernelGGL(sum_chunks, dim3(sum_chunks_blocks(COUT[l])), dim3(256), 0, net.stream, net.part,
+                       net.g + net.off_b[l], chunks,
                        COUT[l]);
     // data gradient into the previous layer (times gelu')
     if (l > 0) {

@@ -25,6 +25,12 @@ H = W = 256
 SUPER = 32            # stencil steps per forward pass
 CHECK_EVERY = 8       # super-steps between checks
 BOUND = 1e-3          # relative L2 bound of a checked super-step
+# The accumulated-error check compares only the shadow instances, a sample of
+# the batch, and the largest error over the batch runs above the sample's
+# (16 held-out trajectories after three windows: 1.18e-3 over all of them
+# against 9.1e-4 on the two shadows); it falls back once the sample reaches
+# this fraction of the bound.
+ACC_MARGIN = 0.8
 F_RANGE = (0.02, 0.055)
 K_RANGE = (0.05, 0.065)
 CIN = (4, 48, 48, 48, 48)
@@ -272,7 +278,7 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 
 def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float = 1e-3, max_seconds: float = 120.0,
           seed: int = 7, passes: int = 1, noise: float = 0.0, store_gib: float = 128.0, unroll: int = 0,
-          log=print) -> TrainResult:
+          init=None, log=print) -> TrainResult:
     """Train on trajectories of the FP32 stencil: generate, then replay.
 
     Generation: all n_traj instances advance together through the batched
@@ -289,7 +295,10 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     through j = 0..unroll passes of the network itself (no gradient), and the
     target is the true state j + 1 super-steps after the start: the network
     learns to steer its own drifted states back to the trajectory.
-    Stops after max_seconds of training."""
+    With init = (theta, scale) training continues from those weights and
+    keeps their output scale (warm start).  A minibatch's states and targets
+    are gathered on the device with one call each.  Stops after max_seconds
+    of training."""
     sur = Surrogate.get(batch)
     specs = trajectory_specs(n_traj, seed)
     st = Stencil(specs)
@@ -322,14 +331,14 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     sur.lib.Sur_Copy(ctypes.c_void_p(bview.ptr), ctypes.c_void_p(snaps.ptr + nb * n_keep), ctypes.c_size_t(nb * n_keep), 2)
     inc = (bview.download((n_keep, 2, H, W)) - a).astype(np.float64)
     bview.free()
-    scale = (float(inc[:, 0].std()), float(inc[:, 1].std()))
+    scale = (float(inc[:, 0].std()), float(inc[:, 1].std())) if init is None else tuple(init[1])
     sur.set_scale(*scale)
     res.scale = scale
     gen_s = time.perf_counter() - t0
     log(f"generated {n_traj} trajectories x {rounds} super-steps ({n_keep} kept, "
         f"{snaps.nbytes / 2 ** 30:.1f} GiB) in {gen_s:.1f} s; output scale {scale}")
     # replay training
-    theta = init_params(seed)
+    theta = init_params(seed) if init is None else np.asarray(init[0], dtype=np.float32)
     sur.set_params(theta)
     rng = np.random.default_rng(seed)
     bstate = sur.buffer(nb * batch)
@@ -340,15 +349,14 @@ def train(n_traj: int = 2000, traj_steps: int = 4096, batch: int = 16, lr: float
     total = max(1, passes * rounds * n_keep // batch)
     t1 = time.perf_counter()
     for step in range(total):
-        # pushforward from the second quarter on (the untrained network's passes are noise)
-        j = int(rng.integers(0, unroll + 1)) if unroll > 0 and step >= total // 4 else 0
+        # pushforward from the second quarter on (the untrained network's passes are noise;
+        # a warm start has no such phase)
+        j = int(rng.integers(0, unroll + 1)) if unroll > 0 and (init is not None or step >= total // 4) else 0
         traj = rng.integers(0, n_keep, batch)
         rnd = rng.integers(0, rounds - j, batch)
-        for q in range(batch):
-            src = snaps.ptr + (int(rnd[q]) * n_keep + int(traj[q])) * nb
-            sur.lib.Sur_Copy(ctypes.c_void_p(bstate.ptr + q * nb), ctypes.c_void_p(src), nb, 2)
-            sur.lib.Sur_Copy(ctypes.c_void_p(btarget.ptr + q * nb), ctypes.c_void_p(src + (j + 1) * n_keep * nb),
-                             nb, 2)
+        src = [snaps.ptr + (int(rnd[q]) * n_keep + int(traj[q])) * nb for q in range(batch)]
+        sur.gather(src, bstate, n_each)
+        sur.gather([p + (j + 1) * n_keep * nb for p in src], btarget, n_each)
         bfk.upload(fk_host[traj])
         for _ in range(j):                 # the network's own passes, no gradient
             sur.forward(bstate, bfk, bspare, batch)
@@ -439,7 +447,8 @@ class Rollout:
 
 
 def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarray, n_super: int,
-                    check_every: int = CHECK_EVERY, bound: float = BOUND, keep_windows: bool = False) -> Rollout:
+                    check_every: int = CHECK_EVERY, bound: float = BOUND, keep_windows: bool = False,
+                    acc_margin: float = ACC_MARGIN) -> Rollout:
     """Advance x0 [B][2][H][W] by n_super super-steps (32 stencil steps each).
 
     Windows of `check_every` super-steps: the CNN advances the window.  Two
@@ -451,8 +460,9 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
         (st.shadows, a sample of the trajectories, advanced by the stencil
         alone from the rollout's start) against the rollout's state of the
         same trajectories after the window; the relative L2 difference must
-        be within `bound` -- this sees the error the unchecked super-steps
-        accumulated, which the local check cannot.
+        be within `acc_margin` x `bound` (the shadows are a sample of the
+        batch) -- this sees the error the unchecked super-steps accumulated,
+        which the local check cannot.
     When a check fails, the window is recomputed from its start state by the
     stencil, which then advances the rest of the rollout."""
     B = x0.shape[0]
@@ -502,7 +512,7 @@ def checked_rollout(sur: Surrogate, st: Stencil, fk: DeviceBuffer, x0: np.ndarra
                 sur.lib.Sur_Copy(ctypes.c_void_p(samp.ptr + q * nb1), ctypes.c_void_p(ref.ptr + i * nb1), nb1, 2)
             acc = max(sur.rel_l2(samp, shad, k, x0[0].size))
             res.accumulated.append(acc)
-        if err <= bound and acc <= bound:
+        if err <= bound and acc <= acc_margin * bound:
             a, ref = ref, a
         else:                                     # roll back and redo the window with the stencil
             res.fallback_at = j
@@ -560,7 +570,8 @@ def evaluate(theta: np.ndarray, scale, n_traj: int = 16, n_super: int = 32, seed
     st.store(sur, start)
     xs = start.download(x0.shape)
     out: Dict = {"trajectories": n_traj, "super_steps": n_super, "steps": steps, "bound": bound,
-                 "check_every": CHECK_EVERY, "shadow_instances": shadow_sample(n_traj)}
+                 "check_every": CHECK_EVERY, "shadow_instances": shadow_sample(n_traj),
+                 "accumulated_check_at": ACC_MARGIN * bound}
     # FP64 reference
     ref64 = open_batch("gs2d_batch_f64")
     ref64.init(specs)
@@ -677,16 +688,28 @@ def main(argv=None):
     ap.add_argument("--unroll", type=int, default=0, help="pushforward: up to this many network passes before the target")
     ap.add_argument("--store-gib", type=float, default=128.0,
                     help="HBM for the kept trajectories (2000 x 127 states of 256^2 x 2 FP32: 124 GiB)")
+    ap.add_argument("--init", default="", help="train: continue from these weights (warm start, same output scale)")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--weights", default=WEIGHTS)
     a = ap.parse_args(argv)
     if a.command == "train":
+        init, prior = None, None
+        if a.init:
+            import ast
+            t0_, s0_, prior_s = load_weights(a.init)
+            init = (t0_, s0_)
+            prior = ast.literal_eval(prior_s)
         res = train(a.trajectories, a.traj_steps, a.batch, a.lr, a.seconds, passes=a.passes, noise=a.noise,
-                    store_gib=a.store_gib, unroll=a.unroll, log=lambda m: print(m, flush=True))
+                    store_gib=a.store_gib, unroll=a.unroll, init=init, log=lambda m: print(m, flush=True))
         theta = Surrogate.get(a.batch).get_params()
         meta = {"trajectories": a.trajectories, "kept": res.kept, "traj_steps": a.traj_steps, "adam_steps": res.steps,
                 "passes": a.passes, "noise": a.noise, "unroll": a.unroll, "lr": a.lr, "batch": a.batch,
                 "seconds": round(res.seconds, 1), "final_loss": float(np.mean(res.losses[-100:]))}
+        if prior is not None:
+            meta["warm_start_from"] = {k: prior[k] for k in ("adam_steps", "seconds", "lr", "unroll", "passes")
+                                       if k in prior}
+            if "warm_start_from" in prior:
+                meta["warm_start_from"]["earlier"] = prior["warm_start_from"]
         save_weights(a.out, theta, res.scale, meta)
         print(json.dumps(meta), flush=True)
         ev = evaluate(theta, res.scale, log=lambda m: print(m, flush=True))

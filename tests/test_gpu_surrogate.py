@@ -213,8 +213,8 @@ def test_accumulated_check_measures_the_drift():
     value must equal (to 1e-6 relative) the host-computed relative L2
     between the rollout's final state of those trajectories and an FP32
     stencil run of them for the same steps; and the rollout either keeps
-    every accumulated error within the bound or falls back at the window
-    where one exceeds it."""
+    every accumulated error within the check's threshold (ACC_MARGIN of the
+    bound) or falls back at the window where one exceeds it."""
     from formura_amd import surrogate as S
     theta, scale, _ = S.load_weights()
     sur = S.Surrogate.get()
@@ -237,8 +237,9 @@ def test_accumulated_check_measures_the_drift():
     ro = S.checked_rollout(sur, sts, fk, xs, n_super)
     _say(f"[surrogate] accumulated errors {ro.accumulated}, local {ro.checks}, fallback {ro.fallback_at}")
     assert len(ro.accumulated) == len(ro.checks)
+    acc_at = S.ACC_MARGIN * S.BOUND     # the accumulated check's threshold (the shadows are a sample)
     if ro.fallback_at is None:
-        assert max(ro.accumulated) <= S.BOUND
+        assert max(ro.accumulated) <= acc_at
         # the stencil alone, from the same start, for the shadowed trajectories
         st2 = S.Stencil(specs)
         buf.upload(xs)
@@ -251,7 +252,7 @@ def test_accumulated_check_measures_the_drift():
         assert abs(ro.accumulated[-1] - host) <= 1e-6 * host + 1e-12
     else:
         w = ro.fallback_at // S.CHECK_EVERY
-        assert ro.accumulated[w] > S.BOUND or ro.checks[w] > S.BOUND
-        assert all(a <= S.BOUND for a in ro.accumulated[:w])
+        assert ro.accumulated[w] > acc_at or ro.checks[w] > S.BOUND
+        assert all(a <= acc_at for a in ro.accumulated[:w])
     for b in (buf, fk):
         b.free()
