@@ -99,7 +99,7 @@ def _run(cmd: List[str]) -> str:
 
 RESOURCE_FLAG = "-Rpass-analysis=kernel-resource-usage"
 _MARCH_NAME = re.compile(r"march3d_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)E")
-_MARCHV_NAME = re.compile(r"marchv_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELb(\d)ELi(\d+)ELi(\d+)E")
+_MARCHV_NAME = re.compile(r"marchv_kernel\w*?ProgELi(\d+)ELi(\d+)ELi(\d+)ELi(\d+)ELb(\d)ELi(\d+)ELi(\d+)ELi(\d+)E")
 
 
 def parse_resources(text: str) -> Dict[str, Dict[str, int]]:
@@ -122,7 +122,7 @@ def parse_resources(text: str) -> Dict[str, Dict[str, int]]:
 def march_resources(res: Dict[str, Dict[str, int]]) -> Dict[tuple, Dict[str, int]]:
     """(kind, T, rows, waves, prefetch, occupancy) -> resources, for the march
     kernels (kind 1: march3d_kernel; 3 / 4: marchv_kernel with one / two
-    z-sites per lane)."""
+    z-sites per lane; 5 / 6: the same with edge rows)."""
     out = {}
     for name, r in res.items():
         m = _MARCH_NAME.search(name)
@@ -131,8 +131,8 @@ def march_resources(res: Dict[str, Dict[str, int]]) -> Dict[tuple, Dict[str, int
             continue
         m = _MARCHV_NAME.search(name)
         if m:
-            T, by, W, vz, _xc, pf, occ = (int(x) for x in m.groups())
-            out[(2 + vz, T, by, W, pf, occ)] = r
+            T, by, W, vz, _xc, pf, occ, ey = (int(x) for x in m.groups())
+            out[(2 + vz + (2 if ey else 0), T, by, W, pf, occ)] = r
     return out
 
 
@@ -164,15 +164,17 @@ def build_program(name: str) -> str:
     spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}
     if spilled:
         from formura_amd.compiler.driver import write_hip
-        from formura_amd.compiler.emit_hip import _with_kind, variants_for
+        from formura_amd.compiler.emit_hip import (KIND_MARCHV1E, KIND_MARCHV2E, MARCH_EDGE_ROWS, _with_kind,
+                                                   variants_for)
         base = [_with_kind(v) for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision))]
         trial = []
         for v in base:
             kind, T, by, W = v[:4]
             fewer = (kind, T, by - 1) + v[3:]
+            edge = kind in (KIND_MARCHV1E, KIND_MARCHV2E)
             if v not in spilled:
                 trial.append(v)
-            elif (by - 1) * W > 2 * T and fewer not in base:
+            elif (by - 1) * W > 2 * T and fewer not in base and (not edge or by - 1 + MARCH_EDGE_ROWS > T):
                 trial.append(fewer)
         _, src = write_hip(c, BUILD, trial)
         out = compile_hip(src, lib_path(name))

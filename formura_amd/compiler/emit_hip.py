@@ -135,7 +135,20 @@ def default_march_variants(nf: int, real: str, max_t: int = None, ny: int = None
 KIND_MARCH = 1       # march3d_kernel
 KIND_MARCHV1 = 3     # marchv_kernel, one z-site per lane
 KIND_MARCHV2 = 4     # marchv_kernel, two z-sites per lane
-MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
+KIND_MARCHV1E = 5    # marchv_kernel, one z-site per lane, edge rows on the first and last wave
+KIND_MARCHV2E = 6    # marchv_kernel, two z-sites per lane, edge rows on the first and last wave
+MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2, KIND_MARCHV1E, KIND_MARCHV2E)
+MARCH_EDGE_ROWS = 2  # extra rows of the first and the last wave (csrc/fmr_library.hpp)
+
+
+def march_vz(kind: int) -> int:
+    """z-sites per lane of a marchv kind."""
+    return 2 if kind in (KIND_MARCHV2, KIND_MARCHV2E) else 1
+
+
+def march_rows(kind: int, by: int, waves: int) -> int:
+    """Rows of a march tile (csrc/fmr_library.hpp::march_rows)."""
+    return by * waves + (2 * MARCH_EDGE_ROWS if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0)
 MAX_MARCHV_LDS = 144 * 1024   # LDS of one workgroup on gfx950 (160 KB per CU)
 
 
@@ -184,6 +197,13 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
                     out += [(kind, T, by - 1, W, 1, 2), (kind, T, by - 1, W, 2, 2)]
                 elif vz == 2 and W * (by - 1) - 2 * T >= 4 * T:
                     out.append((kind, T, by - 1, W, 1, 2))
+                # the one-site tiles with edge rows on the first and the last
+                # wave (the two-site ones, kind 6, gave wrong bits now and then
+                # in the parity tests, cause not found: not generated)
+                if vz == 1:
+                    for b in (by, by - 1):
+                        if b >= 1 and b + MARCH_EDGE_ROWS > T and W * b - 2 * T >= 4 * T:
+                            out.append((KIND_MARCHV1E, T, b, W, 1, 2))
             if ny and by8 >= 1 and marchv_lds_bytes(T, 8, nf, words, vz) <= MAX_MARCHV_LDS:
                 fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
                 if fits and (kind, T, max(fits), 8, 1, 2) not in out:
@@ -403,9 +423,10 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
             w(f"  {{{{fmr::KIND_MARCH, {T}, {by}, {wv}, {pf}, {occ}}}, "
               f"&fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}, {occ}>}},")
         else:
-            vz = 1 if kind == KIND_MARCHV1 else 2
+            vz = march_vz(kind)
+            ey = MARCH_EDGE_ROWS if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0
             w(f"  {{{{{kind}, {T}, {by}, {wv}, {pf}, {occ}}}, "
-              f"&fmr::launch_marchv<Prog, {T}, {by}, {wv}, {vz}, {pf}, {occ}>}},")
+              f"&fmr::launch_marchv<Prog, {T}, {by}, {wv}, {vz}, {pf}, {occ}, {ey}>}},")
     if not variants:
         w("  {{fmr::KIND_DIRECT, 1, 0, 1, 1, 1}, nullptr},")
     w("};")

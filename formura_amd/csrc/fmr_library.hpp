@@ -141,17 +141,28 @@ inline int diff_noalloc_entry(const void* a, int a_elem, const void* ref, int re
 
 // ---- kernel variants --------------------------------------------------------
 // KIND_MARCH: march3d_kernel (fmr_stencil.hpp); KIND_MARCHV1 / KIND_MARCHV2:
-// marchv_kernel (fmr_march.hpp) with one / two z-sites per lane
-enum KernelKind { KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2, KIND_MARCHV1 = 3, KIND_MARCHV2 = 4 };
+// marchv_kernel (fmr_march.hpp) with one / two z-sites per lane;
+// KIND_MARCHV1E / KIND_MARCHV2E: the same with MARCH_EDGE_ROWS more rows on
+// the first and the last wave of a tile (MarchRole)
+enum KernelKind {
+  KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2, KIND_MARCHV1 = 3, KIND_MARCHV2 = 4, KIND_MARCHV1E = 5,
+  KIND_MARCHV2E = 6
+};
+constexpr int MARCH_EDGE_ROWS = 2;
 
-constexpr bool is_march(int kind) { return kind == KIND_MARCH || kind == KIND_MARCHV1 || kind == KIND_MARCHV2; }
+constexpr bool is_march(int kind) { return kind == KIND_MARCH || (kind >= KIND_MARCHV1 && kind <= KIND_MARCHV2E); }
+constexpr bool two_sites(int kind) { return kind == KIND_MARCHV2 || kind == KIND_MARCHV2E; }
+// rows of a march tile of W waves of `by` rows
+constexpr int march_rows(int kind, int by, int W) {
+  return by * W + ((kind == KIND_MARCHV1E || kind == KIND_MARCHV2E) ? 2 * MARCH_EDGE_ROWS : 0);
+}
 
 // z-sites a wavefront stores per row of a tile: 64 * VZ less the ghost sites
 // (march3d_kernel: none at T = 1, T per side otherwise; marchv_kernel: T per
 // side rounded up to whole lanes)
 constexpr int march_stored_z(int kind, int T) {
   return kind == KIND_MARCH ? (T == 1 ? WAVE : WAVE - 2 * T)
-                            : (kind == KIND_MARCHV2 ? 2 * WAVE - 2 * ((T + 1) / 2 * 2) : WAVE - 2 * T);
+                            : (two_sites(kind) ? 2 * WAVE - 2 * ((T + 1) / 2 * 2) : WAVE - 2 * T);
 }
 
 // The z tiling of a march launch over nz sites: stored sites per tile row and
@@ -161,10 +172,10 @@ constexpr int march_stored_z(int kind, int T) {
 // whole lines, none shares a line with its neighbour, and the wider ghost
 // is lanes the tile computes anyway.
 inline void march_tile_z(int kind, int T, int nz, int elem, int& stored, int& ghost) {
-  const int zw = kind == KIND_MARCHV2 ? 2 * WAVE : WAVE;
+  const int zw = two_sites(kind) ? 2 * WAVE : WAVE;
   stored = march_stored_z(kind, T);
   ghost = (zw - stored) / 2;
-  if (kind == KIND_MARCHV2) {
+  if (two_sites(kind)) {
     const int line = 128 / elem;
     const int s1 = stored / line * line;
     if (s1 > 0 && (nz + s1 - 1) / s1 == (nz + stored - 1) / stored) {
@@ -198,7 +209,7 @@ void launch_march3d(Fields<typename P::real> in, Fields<typename P::real> out, i
                      nty, ntz, chunk, prm);
 }
 
-template <class P, int T, int BY, int W, int VZ, int PF, int OCC>
+template <class P, int T, int BY, int W, int VZ, int PF, int OCC, int EY = 0>
 void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int groups,
                    int nty, int ntz, int chunk, int ghost, Params prm, hipStream_t s) {
   if ((size_t)ny * nz * sizeof(typename P::real) >= ((size_t)1 << 31)) {
@@ -216,7 +227,7 @@ void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, in
   // the x-carry with one z-site per lane (P::XC); with two, the window keeps
   // the plane behind (fewer registers at the peak of a row's evaluation)
   constexpr bool XC = VZ == 1 && P::XC;
-  hipLaunchKernelGGL((marchv_kernel<P, T, BY, W, VZ, XC, PF, OCC>), dim3(groups), dim3(WAVE * W), 0, s, in, out,
+  hipLaunchKernelGGL((marchv_kernel<P, T, BY, W, VZ, XC, PF, OCC, EY>), dim3(groups), dim3(WAVE * W), 0, s, in, out,
                      nx, ny, nz, nty, ntz, chunk, ghost, prm);
 }
 
@@ -477,7 +488,7 @@ class StepLibrary {
   }
 
   void launch_march(const VariantEntry<P>* e) {
-    const int T = e->v.T, rows = e->v.by * e->v.waves;
+    const int T = e->v.T, rows = march_rows(e->v.kind, e->v.by, e->v.waves);
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
     int zw = 0, ghost = 0;
     march_tile_z(e->v.kind, T, n[2], (int)sizeof(real), zw, ghost);

@@ -103,13 +103,25 @@ struct VecIO<float, 2> {
   }
 };
 
-template <class P, int T, int BY, int W, int VZ, bool XC, int PF>
+// Wave roles of a multi-wave tile with edge rows (EY > 0): the first wave
+// (ROLE_TOP) and the last (ROLE_BOTTOM) hold BY + EY rows, the others
+// (ROLE_INNER) BY.  Level l (1..T) is needed only on tile rows [l, ROWS - l),
+// so the first wave computes level l on its rows [l, BY + EY) and the last
+// on [0, BY + EY - l): with EY = 2 at T = 4 they evaluate 26 row-levels
+// against the inner waves' 28, and the tile stores 4 rows more for about the
+// same work.  ROLE_ALL (EY = 0): every wave BY rows, all rows at every level
+// (the waves meet at a barrier each plane, so rows the first and last wave
+// skipped would save no time).
+enum MarchRole { ROLE_ALL = 0, ROLE_TOP = 1, ROLE_INNER = 2, ROLE_BOTTOM = 3 };
+
+template <class P, int T, int BY, int W, int VZ, bool XC, int PF, int EY = 0, int ROLE = ROLE_ALL>
 struct MarchV {
   using real = typename P::real;
   static constexpr int NF = P::NF;
   static constexpr int NC = XC ? P::NC : 0;     // carried values per site
   static constexpr int NP = XC ? 2 : 3;         // window planes per level
-  static constexpr int ROWS = BY * W;
+  static constexpr int RB = (ROLE == ROLE_TOP || ROLE == ROLE_BOTTOM) ? BY + EY : BY;   // rows of this wave
+  static constexpr int ROWS = BY * W + 2 * EY;
   static constexpr int G = (T + VZ - 1) / VZ * VZ;   // fewest ghost z-sites per side (whole lanes)
   static constexpr int ZW = WAVE * VZ;               // z-sites per row of a tile
   static_assert(T >= 1 && ROWS > 2 * T, "tile must be taller than its ghost rows");
@@ -117,11 +129,12 @@ struct MarchV {
   static_assert(VZ == 1 || VZ == 2, "one or two z-sites per lane");
   static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
   static_assert(!XC || P::NC > 0, "x-carry needs carried operations");
+  static_assert(EY == 0 ? ROLE == ROLE_ALL : (W > 1 && ROLE != ROLE_ALL && BY + EY > T), "edge rows need roles");
   using Edge = real[2][T][W][2][NF][VZ][WAVE];   // [half][level][wave][top,bottom][field][site][lane]
 
-  real w[T][NP][BY][VZ][NF];
-  real c[T][BY][VZ][NC > 0 ? NC : 1];
-  real nxt[PF][BY][VZ][NF];
+  real w[T][NP][RB][VZ][NF];
+  real c[T][RB][VZ][NC > 0 ? NC : 1];
+  real nxt[PF][RB][VZ][NF];
   Edge* edge;
   int wave, lane, half;
   int wave_above, wave_below;   // (wave - 1) and (wave + 1) modulo W
@@ -131,7 +144,15 @@ struct MarchV {
   int zg;              // wrapped z of this lane's first site
   unsigned zoff;       // byte offset of this lane's first site within a row
   unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane or a lane past the grid end
-  unsigned roff[BY];   // byte offset of row r within a plane (uniform)
+  unsigned roff[RB];   // byte offset of row r within a plane (uniform)
+
+  // tile row of this wave's row 0
+  __device__ __forceinline__ int row0() const {
+    if constexpr (ROLE == ROLE_ALL) return wave * BY;
+    else if constexpr (ROLE == ROLE_TOP) return 0;
+    else if constexpr (ROLE == ROLE_INNER) return BY + EY + (wave - 1) * BY;
+    else return BY + EY + (W - 2) * BY;
+  }
 
   // y-neighbours across a wave boundary: the edge row the neighbouring wave
   // published.  The tile's first and last rows are ghost rows whose values
@@ -145,7 +166,7 @@ struct MarchV {
   }
   template <int L, int CEN, int ROW, int V, int F>
   __device__ __forceinline__ real row_below() const {
-    if constexpr (ROW < BY - 1) return w[L][CEN][ROW + 1][V][F];
+    if constexpr (ROW < RB - 1) return w[L][CEN][ROW + 1][V][F];
     else if constexpr (W == 1) return w[L][CEN][ROW][V][F];
     else return (*edge)[half][L][wave_below][0][F][V][lane];
   }
@@ -207,7 +228,7 @@ struct MarchV {
     for (int f = 0; f < NF; ++f) {
       const auto rs = plane_rsrc(reinterpret_cast<const char*>(in.f[f]) + (size_t)x * plane_bytes, plane_bytes);
 #pragma unroll
-      for (int r = 0; r < BY; ++r) {
+      for (int r = 0; r < RB; ++r) {
         real v[VZ];
         VecIO<real, VZ>::load(v, rs, zoff, roff[r]);
 #pragma unroll
@@ -217,17 +238,18 @@ struct MarchV {
   }
 
   // rows of level L+1 computed from window L: with one wave, the tile's rows
-  // [L+1, ROWS-L-1) (known at compile time); with several, every row of
-  // every wave -- the waves meet at a barrier each plane, so rows the first
-  // and last wave could skip would save no time, only add branches
-  static constexpr int row_lo(int L) { return W == 1 ? L + 1 : 0; }
-  static constexpr int row_hi(int L) { return W == 1 ? BY - L - 1 : BY; }
+  // [L+1, ROWS-L-1) (known at compile time); with several and no edge rows,
+  // every row of every wave -- the waves meet at a barrier each plane, so
+  // rows the first and last wave could skip would save no time, only add
+  // branches; with edge rows, the first and last wave skip them (MarchRole)
+  static constexpr int row_lo(int L) { return (W == 1 || ROLE == ROLE_TOP) ? L + 1 : 0; }
+  static constexpr int row_hi(int L) { return (W == 1 || ROLE == ROLE_BOTTOM) ? RB - L - 1 : RB; }
 
   template <int L, int PH, int ROW>
   __device__ __forceinline__ void level_row(const Params& prm, int q, int y0, const __amdgpu_buffer_rsrc_t (&orsrc)[NF],
                                             bool st_ok) {
     if constexpr (ROW < row_hi(L)) {
-      const int trow = wave * BY + ROW;
+      const int trow = row0() + ROW;
       Acc<L, PH, ROW> a{*this, prm, q, y0 + trow, {}};
       P::template mpoint<VZ>(a);
       if constexpr (L + 1 < T) {
@@ -235,7 +257,7 @@ struct MarchV {
         for (int s = 0; s < VZ; ++s)
 #pragma unroll
           for (int f = 0; f < NF; ++f) w[L + 1][PH][ROW][s][f] = a.out[s][f];
-      } else if (st_ok && (W == 1 || (trow >= T && trow < ROWS - T))) {
+      } else if (st_ok && (W == 1 || ROLE != ROLE_ALL || (trow >= T && trow < ROWS - T))) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           real v[VZ];
@@ -279,7 +301,7 @@ struct MarchV {
                                        int xl, int x0, int xend, int y0, unsigned plane_bytes) {
     constexpr int CEN = (PH + NP - 1) % NP;
 #pragma unroll
-    for (int r = 0; r < BY; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
       for (int s = 0; s < VZ; ++s)
 #pragma unroll
@@ -295,8 +317,10 @@ struct MarchV {
         for (int f = 0; f < NF; ++f)
 #pragma unroll
           for (int s = 0; s < VZ; ++s) {
-            (*edge)[half][l][wave][0][f][s][lane] = w[l][CEN][0][s][f];
-            (*edge)[half][l][wave][1][f][s][lane] = w[l][CEN][BY - 1][s][f];
+            // the first wave's first row and the last wave's last row are
+            // read by no wave when they are edge rows
+            if constexpr (ROLE != ROLE_TOP) (*edge)[half][l][wave][0][f][s][lane] = w[l][CEN][0][s][f];
+            if constexpr (ROLE != ROLE_BOTTOM) (*edge)[half][l][wave][1][f][s][lane] = w[l][CEN][RB - 1][s][f];
           }
       __syncthreads();
     }
@@ -326,8 +350,8 @@ struct MarchV {
     zg = wrap(zu, nz);
     const unsigned plane_bytes = (unsigned)((size_t)ny * nz * sizeof(real));
 #pragma unroll
-    for (int r = 0; r < BY; ++r)
-      roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + wave * BY + r, ny) * nz * (int)sizeof(real));
+    for (int r = 0; r < RB; ++r)
+      roff[r] = (unsigned)__builtin_amdgcn_readfirstlane(wrap(y0 + row0() + r, ny) * nz * (int)sizeof(real));
     zoff = (unsigned)(zg * sizeof(real));
     zst = (lane * VZ >= gz && lane * VZ < ZW - gz && zu < nz) ? zoff : BUF_OOB;
     int xl = wrap(x0 - T, nx);
@@ -342,40 +366,19 @@ struct MarchV {
   }
 };
 
-// Work split and workgroup order as march3d_kernel: x cut into chunks of L
-// planes, (chunk, tile, plane) items listed chunk-major, workgroup g of G
-// marches items [g * total / G, (g + 1) * total / G); consecutive workgroups
-// on one XCD.  Requires nz to be a multiple of VZ (a lane's sites never wrap).
-template <class P, int T, int BY, int W, int VZ, bool XC, int PF, int OCC>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
-marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int nty, int ntz,
-              int L, int gz, Params prm) {
-  using M = MarchV<P, T, BY, W, VZ, XC, PF>;
+// The (chunk, tile, plane) items of workgroup g of G, marched by m.
+template <class M>
+__device__ __forceinline__ void marchv_items(M& m, const Fields<typename M::real>& in,
+                                             const Fields<typename M::real>& out, int nx, int nty, int ntz, int L,
+                                             const Params& prm) {
   const int G = gridDim.x;
-  const int b = blockIdx.x;
-  const int g = xcd_order(b, G);
+  const int g = xcd_order(blockIdx.x, G);
   const long long ntile = (long long)nty * ntz;
   const long long total = ntile * nx;
   const int nfull = nx / L;
   const long long full_items = ntile * nfull * L;
   long long a = (long long)g * total / G;
   const long long e = (long long)(g + 1) * total / G;
-  M m;
-  m.wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-  m.lane = threadIdx.x % WAVE;
-  m.wave_above = m.wave == 0 ? W - 1 : m.wave - 1;
-  m.wave_below = m.wave == W - 1 ? 0 : m.wave + 1;
-  m.half = 0;
-  m.nx = nx;
-  m.ny = ny;
-  m.nz = nz;
-  m.gz = gz;
-  if constexpr (W > 1) {
-    __shared__ typename M::Edge edge_buf;
-    m.edge = &edge_buf;
-  } else {
-    m.edge = nullptr;
-  }
   while (a < e) {
     int c0, len;
     long long r;
@@ -394,6 +397,62 @@ marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx,
     const int xend = (int)min((long long)(c0 + len), x0 + (e - a));
     m.run(in, out, ntz, tile, x0, xend, prm);
     a += xend - x0;
+  }
+}
+
+template <class M, int W>
+__device__ __forceinline__ void marchv_setup(M& m, int wave, int nx, int ny, int nz, int gz, typename M::Edge* edge) {
+  m.wave = wave;
+  m.lane = threadIdx.x % WAVE;
+  m.wave_above = wave == 0 ? W - 1 : wave - 1;
+  m.wave_below = wave == W - 1 ? 0 : wave + 1;
+  m.half = 0;
+  m.nx = nx;
+  m.ny = ny;
+  m.nz = nz;
+  m.gz = gz;
+  m.edge = edge;
+}
+
+// Work split and workgroup order as march3d_kernel: x cut into chunks of L
+// planes, (chunk, tile, plane) items listed chunk-major, workgroup g of G
+// marches items [g * total / G, (g + 1) * total / G); consecutive workgroups
+// on one XCD.  Requires nz to be a multiple of VZ (a lane's sites never wrap).
+// With edge rows (EY > 0) the first and last wave run their own role of the
+// march (MarchRole); every role meets the same barriers in the same order.
+template <class P, int T, int BY, int W, int VZ, bool XC, int PF, int OCC, int EY>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(OCC)))
+marchv_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx, int ny, int nz, int nty, int ntz,
+              int L, int gz, Params prm) {
+  using M0 = MarchV<P, T, BY, W, VZ, XC, PF>;
+  using Edge = typename M0::Edge;
+  Edge* edge = nullptr;
+  if constexpr (W > 1) {
+    __shared__ Edge edge_buf;
+    edge = &edge_buf;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  if constexpr (EY == 0) {
+    M0 m;
+    marchv_setup<M0, W>(m, wave, nx, ny, nz, gz, edge);
+    marchv_items(m, in, out, nx, nty, ntz, L, prm);
+  } else {
+    using MT = MarchV<P, T, BY, W, VZ, XC, PF, EY, ROLE_TOP>;
+    using MI = MarchV<P, T, BY, W, VZ, XC, PF, EY, ROLE_INNER>;
+    using MB = MarchV<P, T, BY, W, VZ, XC, PF, EY, ROLE_BOTTOM>;
+    if (wave == 0) {
+      MT m;
+      marchv_setup<MT, W>(m, wave, nx, ny, nz, gz, edge);
+      marchv_items(m, in, out, nx, nty, ntz, L, prm);
+    } else if (wave == W - 1) {
+      MB m;
+      marchv_setup<MB, W>(m, wave, nx, ny, nz, gz, edge);
+      marchv_items(m, in, out, nx, nty, ntz, L, prm);
+    } else {
+      MI m;
+      marchv_setup<MI, W>(m, wave, nx, ny, nz, gz, edge);
+      marchv_items(m, in, out, nx, nty, ntz, L, prm);
+    }
   }
 }
 

@@ -201,8 +201,10 @@ KIND_DIRECT = 0
 KIND_MARCH = 1       # temporally blocked march, one z-site per lane (march3d_kernel)
 KIND_STAGED = 2      # multi-stage programs: one kernel per manifest node
 KIND_MARCHV1 = 3     # march with the x-carry, one z-site per lane (marchv_kernel)
-KIND_MARCHV2 = 4     # march with the x-carry, two z-sites per lane (marchv_kernel)
-MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2)
+KIND_MARCHV2 = 4     # march, two z-sites per lane (marchv_kernel)
+KIND_MARCHV1E = 5    # as 3, with two more rows on the first and the last wave of a tile
+KIND_MARCHV2E = 6    # as 4, with two more rows on the first and the last wave of a tile
+MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2, KIND_MARCHV1E, KIND_MARCHV2E)
 
 
 def balanced_runs(occ: int) -> int:

@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 
 from formura_amd import build
-from formura_amd.runtime import KIND_DIRECT, KIND_STAGED, MARCH_KINDS, StepLibrary, load_program
+from formura_amd.runtime import (KIND_DIRECT, KIND_STAGED, MARCH_KINDS, StepLibrary, balanced_runs, load_program,
+                                 whole_runs)
 from tests.common import gray_scott_ic, normal
 
 pytestmark = pytest.mark.gpu
@@ -111,9 +112,11 @@ def test_every_kernel_variant_bitwise(name, mk):
     want = oracle_run(name, st, 1)
     variants = [(KIND_DIRECT, 1, 0, 1, 1, 1)] + lib.list_kernels()
     for kind, T, by, wv, pf, occ in variants:
-        # runs of 64 or 7 planes (7: runs cross tile boundaries), and one or two
-        # workgroups per SIMD (runs of a plane or two on these small grids)
-        for lx in ((64, 7, -1, -2) + ((-4,) if occ >= 4 else ()) if kind in MARCH_KINDS else (64,)):
+        # runs of 64 or 7 planes (7: runs cross tile boundaries), balanced
+        # and whole runs, and one or two waves per SIMD with x whole (runs of a
+        # plane or two on these small grids)
+        splits = (64, 7, balanced_runs(occ), whole_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
+        for lx in (splits if kind in MARCH_KINDS else (64,)):
             label = f"{name} kind={kind} T={T} rows={by} waves={wv} pf={pf} occ={occ} lx={lx}"
             _say(f"[parity] {label} steps={steps}")
             lib.set_kernel(kind, T, by, wv, pf, occ)
