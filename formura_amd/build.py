@@ -164,7 +164,7 @@ def build_program(name: str) -> str:
     spilled = {k for k, r in resources(name).items() if r.get("ScratchSize", 0) > 0}
     if spilled:
         from formura_amd.compiler.driver import write_hip
-        from formura_amd.compiler.emit_hip import (KIND_MARCHV1E, KIND_MARCHV2E, MARCH_EDGE_ROWS, _with_kind,
+        from formura_amd.compiler.emit_hip import (KIND_MARCHV1E, KIND_MARCHV2E, _with_kind, march_edge_rows,
                                                    variants_for)
         base = [_with_kind(v) for v in (PROGRAMS[name][5] or variants_for(c.layout, c.precision))]
         trial = []
@@ -174,7 +174,7 @@ def build_program(name: str) -> str:
             edge = kind in (KIND_MARCHV1E, KIND_MARCHV2E)
             if v not in spilled:
                 trial.append(v)
-            elif (by - 1) * W > 2 * T and fewer not in base and (not edge or by - 1 + MARCH_EDGE_ROWS > T):
+            elif (by - 1) * W > 2 * T and fewer not in base and (not edge or by - 1 + march_edge_rows(T) > T):
                 trial.append(fewer)
         _, src = write_hip(c, BUILD, trial)
         out = compile_hip(src, lib_path(name))

@@ -138,7 +138,11 @@ KIND_MARCHV2 = 4     # marchv_kernel, two z-sites per lane
 KIND_MARCHV1E = 5    # marchv_kernel, one z-site per lane, edge rows on the first and last wave
 KIND_MARCHV2E = 6    # marchv_kernel, two z-sites per lane, edge rows on the first and last wave
 MARCH_KINDS = (KIND_MARCH, KIND_MARCHV1, KIND_MARCHV2, KIND_MARCHV1E, KIND_MARCHV2E)
-MARCH_EDGE_ROWS = 2  # extra rows of the first and the last wave (csrc/fmr_library.hpp)
+def march_edge_rows(T: int) -> int:
+    """Extra rows of the first and the last wave of an edge-row tile at fusion
+    depth T (csrc/fmr_library.hpp::march_edge_rows): their row-levels then
+    match the inner waves'."""
+    return (T + 1) // 2
 
 
 def march_vz(kind: int) -> int:
@@ -146,9 +150,9 @@ def march_vz(kind: int) -> int:
     return 2 if kind in (KIND_MARCHV2, KIND_MARCHV2E) else 1
 
 
-def march_rows(kind: int, by: int, waves: int) -> int:
+def march_rows(kind: int, T: int, by: int, waves: int) -> int:
     """Rows of a march tile (csrc/fmr_library.hpp::march_rows)."""
-    return by * waves + (2 * MARCH_EDGE_ROWS if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0)
+    return by * waves + (2 * march_edge_rows(T) if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0)
 MAX_MARCHV_LDS = 144 * 1024   # LDS of one workgroup on gfx950 (160 KB per CU)
 
 
@@ -170,7 +174,7 @@ def marchv_lds_bytes(T: int, W: int, nf: int, words: int, vz: int) -> int:
 
 def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
     """(kind, T, rows per wave, waves, prefetch depth, waves per SIMD) of
-    marchv_kernel for the fusion depths where it competes (FP64 T = 3, 4;
+    marchv_kernel for the fusion depths where it competes (FP64 T = 3..5;
     FP32 T = 3..6), at two waves per SIMD (256 VGPRs, no AGPR moves, a second
     wave to hide latency) in workgroups of 8 waves (a CU): one and two z-sites
     per lane (two only when nz is even), as many rows per wave as the
@@ -182,7 +186,7 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
     for 5), and the tile whose stored rows are the largest divisor of ny
     that fits."""
     words = 2 if real == "double" else 1
-    depths = range(3, 5) if real == "double" else range(3, 7)
+    depths = range(3, 6) if real == "double" else range(3, 7)
     out = []
     for T in depths:
         for vz, kind in ((1, KIND_MARCHV1), (2, KIND_MARCHV2)):
@@ -202,7 +206,7 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
                 # in the parity tests, cause not found: not generated)
                 if vz == 1:
                     for b in (by, by - 1):
-                        if b >= 1 and b + MARCH_EDGE_ROWS > T and W * b - 2 * T >= 4 * T:
+                        if b >= 1 and b + march_edge_rows(T) > T and W * b - 2 * T >= 4 * T:
                             out.append((KIND_MARCHV1E, T, b, W, 1, 2))
             if ny and by8 >= 1 and marchv_lds_bytes(T, 8, nf, words, vz) <= MAX_MARCHV_LDS:
                 fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
@@ -424,7 +428,7 @@ def emit_hip(lay: Layout, ir: PointIR, precision: str = "faithful",
               f"&fmr::launch_march3d<Prog, {T}, {by}, {wv}, {pf}, {occ}>}},")
         else:
             vz = march_vz(kind)
-            ey = MARCH_EDGE_ROWS if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0
+            ey = march_edge_rows(T) if kind in (KIND_MARCHV1E, KIND_MARCHV2E) else 0
             w(f"  {{{{{kind}, {T}, {by}, {wv}, {pf}, {occ}}}, "
               f"&fmr::launch_marchv<Prog, {T}, {by}, {wv}, {vz}, {pf}, {occ}, {ey}>}},")
     if not variants:

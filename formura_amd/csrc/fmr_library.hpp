@@ -142,19 +142,22 @@ inline int diff_noalloc_entry(const void* a, int a_elem, const void* ref, int re
 // ---- kernel variants --------------------------------------------------------
 // KIND_MARCH: march3d_kernel (fmr_stencil.hpp); KIND_MARCHV1 / KIND_MARCHV2:
 // marchv_kernel (fmr_march.hpp) with one / two z-sites per lane;
-// KIND_MARCHV1E / KIND_MARCHV2E: the same with MARCH_EDGE_ROWS more rows on
-// the first and the last wave of a tile (MarchRole)
+// KIND_MARCHV1E / KIND_MARCHV2E: the same with march_edge_rows(T) more rows
+// on the first and the last wave of a tile (MarchRole)
 enum KernelKind {
   KIND_DIRECT = 0, KIND_MARCH = 1, KIND_STAGED = 2, KIND_MARCHV1 = 3, KIND_MARCHV2 = 4, KIND_MARCHV1E = 5,
   KIND_MARCHV2E = 6
 };
-constexpr int MARCH_EDGE_ROWS = 2;
+// Extra rows of the first and the last wave at fusion depth T: (T + 1) / 2,
+// which makes their row-levels (T (BY + EY) - T (T + 1) / 2, level l skipped
+// on the l rows nearest the tile edge) match the inner waves' T BY.
+constexpr int march_edge_rows(int T) { return (T + 1) / 2; }
 
 constexpr bool is_march(int kind) { return kind == KIND_MARCH || (kind >= KIND_MARCHV1 && kind <= KIND_MARCHV2E); }
 constexpr bool two_sites(int kind) { return kind == KIND_MARCHV2 || kind == KIND_MARCHV2E; }
 // rows of a march tile of W waves of `by` rows
-constexpr int march_rows(int kind, int by, int W) {
-  return by * W + ((kind == KIND_MARCHV1E || kind == KIND_MARCHV2E) ? 2 * MARCH_EDGE_ROWS : 0);
+constexpr int march_rows(int kind, int T, int by, int W) {
+  return by * W + ((kind == KIND_MARCHV1E || kind == KIND_MARCHV2E) ? 2 * march_edge_rows(T) : 0);
 }
 
 // z-sites a wavefront stores per row of a tile: 64 * VZ less the ghost sites
@@ -488,7 +491,7 @@ class StepLibrary {
   }
 
   void launch_march(const VariantEntry<P>* e) {
-    const int T = e->v.T, rows = march_rows(e->v.kind, e->v.by, e->v.waves);
+    const int T = e->v.T, rows = march_rows(e->v.kind, T, e->v.by, e->v.waves);
     const int nty = (n[1] + (rows - 2 * T) - 1) / (rows - 2 * T);
     int zw = 0, ghost = 0;
     march_tile_z(e->v.kind, T, n[2], (int)sizeof(real), zw, ghost);

@@ -60,9 +60,9 @@ int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
  * `waves` wavefronts per tile, input planes prefetched `pf` (1 or 2) planes
  * ahead, compiled for at least `occ` waves per SIMD; kinds 3 and 4 = the same
  * march with the x-difference carried from plane to plane and one z-site per
- * lane (3), or two consecutive z-sites per lane (4); kind 5 = kind 3 with two
- * more rows on the first and the last wave of a tile, which skip the ghost
- * rows no later level needs.  Returns 0, or -1 if that variant is not
+ * lane (3), or two consecutive z-sites per lane (4); kind 5 = kind 3 with
+ * (T + 1) / 2 more rows on the first and the last wave of a tile, which skip
+ * the ghost rows no later level needs.  Returns 0, or -1 if that variant is not
  * compiled into the library. */
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
