@@ -208,6 +208,8 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
                     for b in (by, by - 1):
                         if b >= 1 and b + march_edge_rows(T) > T and W * b - 2 * T >= 4 * T:
                             out.append((KIND_MARCHV1E, T, b, W, 1, 2))
+                            if b == by - 1:   # and prefetching two planes (+2 % at T = 4, within noise)
+                                out.append((KIND_MARCHV1E, T, b, W, 2, 2))
             if ny and by8 >= 1 and marchv_lds_bytes(T, 8, nf, words, vz) <= MAX_MARCHV_LDS:
                 fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
                 if fits and (kind, T, max(fits), 8, 1, 2) not in out:
