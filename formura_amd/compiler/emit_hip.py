@@ -214,6 +214,27 @@ def default_marchv_variants(nf: int, real: str, ny: int = None, nz: int = None):
                 fits = [by for by in range(1, by8 + 1) if 8 * by - 2 * T >= 4 * T and ny % (8 * by - 2 * T) == 0]
                 if fits and (kind, T, max(fits), 8, 1, 2) not in out:
                     out.append((kind, T, max(fits), 8, 1, 2))
+    if nz == 64:
+        out += [v for v in whole_line_marchv_variants(nf, real) if v not in out]
+    return out
+
+
+def whole_line_marchv_variants(nf: int, real: str):
+    """Small tiles of marchv_kernel for grids whose z-lines are one wavefront
+    long (nz = 64, one site per lane): a wavefront row then holds a whole
+    periodic z-line (the lane moves rotate), so the tile has no ghost lanes,
+    and small grids such as 64^3 need many short tiles to fill the chip.  T = 1
+    to 3 in tiles of 1, 2 or 4 waves whose stored rows (W * rows - 2T) are 2 to
+    16, at two waves per SIMD, within the register model."""
+    words = 2 if real == "double" else 1
+    out = []
+    for T, tiles in ((1, ((6, 1), (10, 1))), (2, ((6, 1), (4, 2), (5, 4))), (3, ((4, 2), (4, 4)))):
+        sites = int((MARCHV_REG_BUDGET - 24) // (nf * words * marchv_regs_per_word(T, 1)))
+        for by, W in tiles:
+            if by <= sites and marchv_lds_bytes(T, W, nf, words, 1) <= MAX_MARCHV_LDS:
+                out.append((KIND_MARCHV1, T, by, W, 1, 2))
+                if T == 2 and W == 1:
+                    out.append((KIND_MARCHV1, T, by, W, 2, 2))
     return out
 
 

@@ -173,12 +173,17 @@ constexpr int march_stored_z(int kind, int T) {
 // down to whole 128-byte lines when that takes no more tiles (nz = 512 in
 // FP64: 112 instead of 120, five tiles either way): every tile then stores
 // whole lines, none shares a line with its neighbour, and the wider ghost
-// is lanes the tile computes anyway.
+// is lanes the tile computes anyway.  marchv_kernel with one site per lane
+// and nz = 64: one tile of whole periodic z-lines (its lane moves rotate), no
+// ghost sites.
 inline void march_tile_z(int kind, int T, int nz, int elem, int& stored, int& ghost) {
   const int zw = two_sites(kind) ? 2 * WAVE : WAVE;
   stored = march_stored_z(kind, T);
   ghost = (zw - stored) / 2;
-  if (two_sites(kind)) {
+  if (kind != KIND_MARCH && !two_sites(kind) && nz == WAVE) {
+    stored = zw;
+    ghost = 0;
+  } else if (two_sites(kind)) {
     const int line = 128 / elem;
     const int s1 = stored / line * line;
     if (s1 > 0 && (nz + s1 - 1) / s1 == (nz + stored - 1) / stored) {
@@ -223,7 +228,8 @@ void launch_marchv(Fields<typename P::real> in, Fields<typename P::real> out, in
     fprintf(stderr, "formura_amd: march kernel with %d z-sites per lane needs NZ divisible by %d\n", VZ, VZ);
     abort();
   }
-  if (ghost < (T + VZ - 1) / VZ * VZ || ghost % VZ != 0 || 2 * ghost >= WAVE * VZ) {
+  if (ghost == 0 ? (VZ != 1 || nz != WAVE)
+                 : (ghost < (T + VZ - 1) / VZ * VZ || ghost % VZ != 0 || 2 * ghost >= WAVE * VZ)) {
     fprintf(stderr, "formura_amd: march kernel: bad ghost width %d\n", ghost);
     abort();
   }

@@ -9,9 +9,15 @@
 // to whole lanes) z-sites on each side are recomputed, and loads wrap
 // periodically.  What differs:
 //
+//  * z-lines held whole.  The lane moves are rotations, so with one site per
+//    lane and NZ = 64 a wavefront row is a whole periodic z-line: the tiling
+//    has one z-tile with no ghost lanes (the host passes gz = 0) and every
+//    lane stores.  Otherwise the wrapped lane is a ghost lane, as the zero
+//    of a shift was.
+//
 //  * VZ consecutive z-sites per lane (1 or 2).  A lane loads and stores its
 //    VZ sites with one 8 * VZ-byte (FP64) access, and of the 2 VZ z-neighbours
-//    of its sites only two come from other lanes (DPP wave shifts): with
+//    of its sites only two come from other lanes (DPP wave rotations): with
 //    VZ = 2 a site costs half the cross-lane moves and half the memory
 //    instructions of VZ = 1.  The point function of the program is emitted
 //    for VZ sites at once (`P::template mpoint<VZ>`), so that a value one
@@ -140,7 +146,7 @@ struct MarchV {
   int wave_above, wave_below;   // (wave - 1) and (wave + 1) modulo W
   int nx, ny, nz;
   int gz;              // ghost z-sites per side: G, or more (a multiple of VZ) so that the stored
-                       // range of every tile starts on a 128-byte line
+                       // range of every tile starts on a 128-byte line, or 0 for whole lines (VZ = 1, NZ = 64)
   int zg;              // wrapped z of this lane's first site
   unsigned zoff;       // byte offset of this lane's first site within a row
   unsigned zst;        // store offset: zoff, or BUF_OOB for a ghost lane or a lane past the grid end
@@ -192,13 +198,14 @@ struct MarchV {
         constexpr int S = V + D2;
         if constexpr (S >= 0 && S < VZ) return m.w[L][CEN][ROW][S][F];
         else if constexpr (S < 0) return lane_prev(m.w[L][CEN][ROW][VZ - 1][F]);
-        else return lane_next(m.w[L][CEN][ROW][0][F]);
+        else return fmr::lane_rot_next(m.w[L][CEN][ROW][0][F]);
       } else {
         return m.w[L][CEN][ROW][V][F];
       }
     }
-    // the value of the previous lane (the z-site before this lane's first)
-    __device__ __forceinline__ real lane_prev(real v) const { return fmr::lane_prev(v); }
+    // the value of the previous lane (the z-site before this lane's first;
+    // lane 0 takes lane 63's)
+    __device__ __forceinline__ real lane_prev(real v) const { return fmr::lane_rot_prev(v); }
     // carried operation K of site V: its value at the previous plane
     template <int K, int V>
     __device__ __forceinline__ real xprev() const {

@@ -96,6 +96,30 @@ __device__ __forceinline__ double lane_next(double v) {
   const int hi = dpp_shl1((int)(q >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+
+// ---- the same moves as rotations (DPP wave_ror / wave_rol): lane 0 receives
+// lane 63 (ror) and lane 63 lane 0 (rol).  When a wavefront holds a whole
+// periodic z-line these are the periodic z-neighbours; otherwise the wrapped
+// lane is a ghost lane like the zero of the shift.  Every lane has a source,
+// so bound_ctrl changes no value; set, it spares the compiler a zeroed
+// destination register per move.
+__device__ __forceinline__ int dpp_ror1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_rol1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, true); }
+
+__device__ __forceinline__ float lane_rot_prev(float v) { return __int_as_float(dpp_ror1(__float_as_int(v))); }
+__device__ __forceinline__ float lane_rot_next(float v) { return __int_as_float(dpp_rol1(__float_as_int(v))); }
+__device__ __forceinline__ double lane_rot_prev(double v) {
+  const long long q = __double_as_longlong(v);
+  const int lo = dpp_ror1((int)(q & 0xffffffffLL));
+  const int hi = dpp_ror1((int)(q >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_rot_next(double v) {
+  const long long q = __double_as_longlong(v);
+  const int lo = dpp_rol1((int)(q & 0xffffffffLL));
+  const int hi = dpp_rol1((int)(q >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 // the value of lane l (uniform l) in every lane
 __device__ __forceinline__ float read_lane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
