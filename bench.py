@@ -132,6 +132,20 @@ def autotune(lib, probe_steps=24, final_steps=120, finalists=4):
     return best
 
 
+def spin_up(lib, var, lx, seconds=2.0):
+    """Run the kernel (var, lx) for about `seconds` of wall time, untimed: the
+    shader clock of an idle GPU takes a while to rise, which the search's
+    timings otherwise do before the timed region.  For runs that skip the
+    search (--variant)."""
+    lib.set_kernel(*var)
+    lib.set_chunk(lx)
+    t0 = time.perf_counter()
+    n = max(var[1], 8)
+    while time.perf_counter() - t0 < seconds:
+        lib.forward_device(n)
+        n = min(2 * n, 4096)
+
+
 def timed_run(lib, var, lx, steps, warmup):
     """Warm up, then time exactly `steps` steps (device synchronised on both
     sides).  Returns (wall seconds, GPU event milliseconds, milliseconds of the
@@ -207,11 +221,15 @@ EXTRA_PROGRAMS = (("heat512_f32", "heat512_fp32"), ("gs3d512", "grayscott512_fp6
                   ("fveuler", "fv_euler256_ssprk3_fp64"), ("mhd", "mhd_glm128_ssprk3_fp64"))
 
 
-def program_extras(only=None, libs=None):
+GRAPHS = True   # --no-graphs: eager launches (for tracers that cannot follow graph replays)
+
+
+def program_extras(only=None, libs=None, variant=None):
     """Autotuned rate of each program of EXTRA_PROGRAMS (or only the one
-    labelled `only`); multi-stage programs also report their stage kernels.
-    The loaded libraries are left in `libs` (name -> library), each set to
-    the kernel it was timed with."""
+    labelled `only`, and with `variant` = ((kind, T, rows, waves, pf, occ), lx)
+    that kernel instead of the search's pick); multi-stage programs also
+    report their stage kernels.  The loaded libraries are left in `libs`
+    (name -> library), each set to the kernel it was timed with."""
     import numpy as np
     from formura_amd.runtime import load_program
     extra = {}
@@ -221,12 +239,20 @@ def program_extras(only=None, libs=None):
             continue
         lg = load_program(name)
         lg.init()
+        lg.set_graphs(GRAPHS)
         libs[name] = lg
         lg.set_state(seeded_state(lg, 1))
         lg.upload()
         n = int(np.prod(lg.layout.intra))
         big = n >= 2 ** 24 or lg.num_stages() > 0
-        best = autotune(lg, probe_steps=24 if big else 200, final_steps=96 if big else 1000)
+        if variant:
+            if variant not in candidates(lg):
+                print(f"bench.py: variant {variant} of {name} is not one the parity tests run", file=sys.stderr)
+                sys.exit(2)
+            spin_up(lg, *variant)
+            best = {variant[0][1]: (0.0,) + tuple(variant)}
+        else:
+            best = autotune(lg, probe_steps=24 if big else 200, final_steps=96 if big else 1000)
         r, v, l = max(best.values())
         steps = 100 if big else 1000
         w, _, _, _ = timed_run(lg, v, l, steps, 10)
@@ -376,16 +402,25 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
-    ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,occ,lx: skip the search")
+    ap.add_argument("--variant", default="", help="kind,T,rows,waves,pf,occ,lx: skip the search (of heat 512^3, "
+                                                 "or with --only of that program)")
     ap.add_argument("--only", default="", help="run only this line of 'extra' (e.g. for a profile)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch eagerly instead of replaying hipGraphs (same kernels; for kernel tracers)")
     a = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not a.no_graphs
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
               "not a measured path", file=sys.stderr)
         sys.exit(2)
 
     if a.only:
-        print(json.dumps({"only": a.only, "extra": program_extras(a.only)}), flush=True)
+        pick = None
+        if a.variant:
+            v = [int(x) for x in a.variant.split(",")]
+            pick = (tuple(v[:6]), v[6])
+        print(json.dumps({"only": a.only, "extra": program_extras(a.only, variant=pick)}), flush=True)
         return
 
     import numpy as np
@@ -394,6 +429,7 @@ def main():
     peak_copy = copy_peak() if not a.variant else None   # before this process's own GPU work
     lib = load_program("heat512")
     lib.init()
+    lib.set_graphs(GRAPHS)
     cells = int(np.prod(lib.layout.intra))
     lib.set_state(seeded_state(lib, 1))
     lib.upload()
@@ -405,6 +441,7 @@ def main():
             print(f"bench.py: variant {a.variant} is not one the parity tests run "
                   f"(a compiled march variant with a split of runtime.march_chunks)", file=sys.stderr)
             sys.exit(2)
+        spin_up(lib, var, lx)
     else:
         log("autotuning heat512 FP64 ...")
         best = autotune(lib)
