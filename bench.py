@@ -493,8 +493,10 @@ def main():
         "config": {"workload": "heat3d 7-point, 512^3 periodic, FP64, D=1 dx=1 dt=0.1",
                    "grid": [512, 512, 512], "kernel": {"kind": var[0], "T": var[1], "rows_per_wave": var[2],
                                                        "waves": var[3], "prefetch": var[4], "waves_per_simd": var[5],
-                                                       "march_chunk": lx, "march_chunk_note": "lx > 0: planes per workgroup run; lx < -200: whole runs, k per tile, "
+                                                       "march_chunk": lx, "march_chunk_note": "lx > 0: planes per workgroup run; -300 <= lx < -200: whole runs, k per tile, "
                                                                            "one resident round at (-lx - 200) waves per SIMD; "
+                                                                           "lx < -300: whole runs at (-lx - 300) waves per SIMD, "
+                                                                           "the idle resident workgroups taking the last planes of every tile; "
                                                                            "-200 <= lx < -100: balanced runs, "
                                                                            "(-lx - 100) waves per SIMD, one run per workgroup; "
                                                                            "other lx < 0: x whole, -lx waves per SIMD"}},

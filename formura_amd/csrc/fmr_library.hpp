@@ -7,6 +7,7 @@
 #include "fmr_march.hpp"
 #include "fmr_stencil.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace fmr {
@@ -457,43 +458,10 @@ class StepLibrary {
     }
   }
 
-  // work split of a march launch (see march3d_kernel): lx > 0 cuts x into
-  // chunks of lx planes, one workgroup per (chunk, tile); -200 > lx cuts x
-  // into the most chunks k whose k * ntile workgroups are resident at once
-  // with (-lx - 200) waves per SIMD (at least one), one workgroup per (chunk,
-  // tile): every workgroup one run, all of them in step; -100 > lx launches
-  // (-lx - 100) waves per SIMD of the device and cuts x into the chunks that
-  // give each workgroup one run of its own (ntile * nx / workgroups planes;
-  // with as many workgroups as are resident at once, every tile of a chunk
-  // starts together and no second round of workgroups is left part-full);
-  // other lx < 0 keep x whole and launch -lx waves per SIMD, balanced to a
-  // plane.  The kernels spread any workgroup count over the XCDs in
-  // contiguous ranges (xcd_order), so the count is exactly the number of
-  // runs wherever the split is one run per workgroup.
-  void march_split(int nty, int ntz, int W, int& groups, int& chunk) const {
-    const long long ntile = (long long)nty * ntz;
-    long long g;
-    if (lx > 0) {
-      chunk = lx < n[0] ? lx : n[0];
-      g = ntile * ((n[0] + chunk - 1) / chunk);
-    } else if (lx < -200) {
-      const long long res = (long long)(-lx - 200) * simds / W;
-      long long k = res / ntile;
-      if (k < 1) k = 1;
-      if (k > n[0]) k = n[0];
-      chunk = (int)((n[0] + k - 1) / k);
-      g = ntile * ((n[0] + chunk - 1) / chunk);
-    } else if (lx < -100) {
-      g = (long long)(-lx - 100) * simds / W;
-      if (g > ntile * n[0]) g = ntile * n[0];
-      const long long c = (ntile * n[0] + g - 1) / g;
-      chunk = (int)(c < 1 ? 1 : (c > n[0] ? n[0] : c));
-    } else {
-      chunk = n[0];
-      g = (long long)(-lx) * simds / W;
-      if (g > ntile * n[0]) g = ntile * n[0];
-    }
-    groups = (int)(g < 1 ? 1 : g);
+  // work split of a march launch (march_split_of, with the library's lx and
+  // the device's SIMD count)
+  void march_split(int nty, int ntz, int W, int T, int& groups, int& chunk) const {
+    march_split_of((long long)nty * ntz, n[0], W, T, lx, simds, groups, chunk);
   }
 
   void launch_march(const VariantEntry<P>* e) {
@@ -503,7 +471,7 @@ class StepLibrary {
     march_tile_z(e->v.kind, T, n[2], (int)sizeof(real), zw, ghost);
     const int ntz = (n[2] + zw - 1) / zw;
     int groups = 0, chunk = 0;
-    march_split(nty, ntz, e->v.waves, groups, chunk);
+    march_split(nty, ntz, e->v.waves, T, groups, chunk);
     e->fn(buf[cur], buf[1 - cur], n[0], n[1], n[2], groups, nty, ntz, chunk, ghost, prm, stream);
     FMR_CHECK(hipGetLastError());
     cur = 1 - cur;

@@ -380,31 +380,8 @@ __device__ __forceinline__ void marchv_items(M& m, const Fields<typename M::real
                                              const Params& prm) {
   const int G = gridDim.x;
   const int g = xcd_order(blockIdx.x, G);
-  const long long ntile = (long long)nty * ntz;
-  const long long total = ntile * nx;
-  const int nfull = nx / L;
-  const long long full_items = ntile * nfull * L;
-  long long a = (long long)g * total / G;
-  const long long e = (long long)(g + 1) * total / G;
-  while (a < e) {
-    int c0, len;
-    long long r;
-    if (a < full_items) {
-      const long long per_chunk = ntile * L;
-      c0 = (int)(a / per_chunk) * L;
-      len = L;
-      r = a % per_chunk;
-    } else {
-      c0 = nfull * L;
-      len = nx - c0;
-      r = a - full_items;
-    }
-    const int tile = (int)(r / len);
-    const int x0 = c0 + (int)(r % len);
-    const int xend = (int)min((long long)(c0 + len), x0 + (e - a));
-    m.run(in, out, ntz, tile, x0, xend, prm);
-    a += xend - x0;
-  }
+  march_runs(g, G, (long long)nty * ntz, nx, L,
+             [&](int tile, int x0, int xend) { m.run(in, out, ntz, tile, x0, xend, prm); });
 }
 
 template <class M, int W>

@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -60,6 +61,126 @@ constexpr int N_XCD = 8;   // MI355X: 8 accelerator complex dies, each with its 
 __device__ __forceinline__ int xcd_order(int b, int G) {
   const int x = b % N_XCD, q = G / N_XCD, rem = G % N_XCD;
   return x * q + (x < rem ? x : rem) + b / N_XCD;
+}
+
+// Work items of a march launch: (chunk, tile, plane), listed chunk-major, x
+// cut into chunks of L planes and a remainder chunk.  march_item_range gives
+// workgroup g of G its items [a, e) and returns L positive.  L > 0: an even
+// share, [g * total / G, (g + 1) * total / G).  L < 0 (whole runs with spare
+// workgroups): chunks of -L planes; the first R = ntile * (nx / -L)
+// workgroups take one whole run of a full chunk each, the other G - R (at
+// least one whenever g >= R) share the remainder chunk's items evenly.
+__host__ __device__ __forceinline__ void march_item_range(int g, int G, long long ntile, int nx, int& L,
+                                                          long long& a, long long& e) {
+  const long long total = ntile * nx;
+  if (L > 0) {
+    a = (long long)g * total / G;
+    e = (long long)(g + 1) * total / G;
+    return;
+  }
+  L = -L;
+  const long long R = ntile * (nx / L);
+  if (g < R) {
+    a = (long long)g * L;
+    e = a + L;
+  } else {
+    const long long full = R * L, rem = total - full, S = G - R;
+    a = full + (g - R) * rem / S;
+    e = full + (g - R + 1) * rem / S;
+  }
+}
+
+// The runs of workgroup g of G: fn(tile, x0, xend) for each stretch of
+// consecutive planes [x0, xend) of one tile within one chunk, in item order.
+// Everything here is uniform over the workgroup, so every wave runs the same
+// runs and reaches the same barriers.
+template <class Fn>
+__host__ __device__ __forceinline__ void march_runs(int g, int G, long long ntile, int nx, int L, Fn&& fn) {
+  long long a, e;
+  march_item_range(g, G, ntile, nx, L, a, e);
+  const int nfull = nx / L;                              // chunks of exactly L planes
+  const long long full_items = ntile * nfull * L;
+  while (a < e) {
+    int c0, len;         // first plane and length of the item's chunk
+    long long r;         // item index within the chunk
+    if (a < full_items) {
+      const long long per_chunk = ntile * L;
+      c0 = (int)(a / per_chunk) * L;
+      len = L;
+      r = a % per_chunk;
+    } else {
+      c0 = nfull * L;
+      len = nx - c0;
+      r = a - full_items;
+    }
+    const int tile = (int)(r / len);
+    const int x0 = c0 + (int)(r % len);
+    const long long stop = x0 + (e - a);
+    const int xend = stop < c0 + len ? (int)stop : c0 + len;
+    fn(tile, x0, xend);
+    a += xend - x0;
+  }
+}
+
+// Work split of a march launch over ntile tiles of nx planes with W-wave
+// workgroups at fusion depth T on a device of `simds` SIMDs: the workgroup
+// count and the chunk argument of the kernels (march_item_range).  lx > 0:
+// x cut into chunks of lx planes, one workgroup per (chunk, tile).
+// -200 > lx >= -300: x cut into the most chunks k whose k * ntile workgroups
+// are resident at once with (-lx - 200) waves per SIMD (at least one), one
+// workgroup per (chunk, tile): every workgroup one run, all of them in step.
+// -300 > lx: the same with (-lx - 300) waves per SIMD, and when it leaves
+// resident workgroups idle (k * ntile short of the count), those take the
+// planes at the end of every tile: the k runs per tile shrink to L planes
+// and the spare workgroups share the ntile * (nx - k L) planes left over
+// (chunk -L), L chosen so that the longer of a run (L + 2T iterations) and a
+// spare workgroup's share (its planes plus 2T per run it starts) is
+// shortest.  -100 > lx >= -200: (-lx - 100) waves per SIMD of the device, x
+// cut into the chunks that give each workgroup one run of its own (ntile *
+// nx / workgroups planes).  Other lx < 0: x whole, -lx waves per SIMD,
+// balanced to a plane.  lx = 0 is not a split.
+inline void march_split_of(long long ntile, int nx, int W, int T, int lx, int simds, int& groups, int& chunk) {
+  long long g;
+  if (lx < -200) {
+    const long long res = (long long)(lx < -300 ? -lx - 300 : -lx - 200) * simds / W;
+    long long k = res / ntile;
+    if (k < 1) k = 1;
+    if (k > nx) k = nx;
+    chunk = (int)((nx + k - 1) / k);
+    g = ntile * ((nx + chunk - 1) / chunk);
+    const long long spare = res - ntile * k;
+    if (lx < -300 && spare > 0) {
+      long long best = chunk + 2 * T;
+      int best_len = 0;
+      for (int len = (int)(nx / k); len > 0 && (k + 1) * len > nx; --len) {
+        const long long left = nx - k * len;
+        const long long share = (ntile * left + spare - 1) / spare;
+        const long long runs = left > 0 ? (share + left - 1) / left + 1 : 0;
+        const long long cost = std::max<long long>(len + 2 * T, share + 2 * T * runs);
+        if (cost < best) {
+          best = cost;
+          best_len = len;
+        }
+      }
+      if (best_len > 0) {
+        chunk = -best_len;
+        g = res;
+      }
+    }
+  } else if (lx > 0) {
+    chunk = lx < nx ? lx : nx;
+    g = ntile * ((nx + chunk - 1) / chunk);
+  } else if (lx < -100) {
+    g = (long long)(-lx - 100) * simds / W;
+    if (g > ntile * nx) g = ntile * nx;
+    const long long c = (ntile * nx + g - 1) / g;
+    chunk = (int)(c < 1 ? 1 : (c > nx ? nx : c));
+  } else {
+    chunk = nx;
+    g = (long long)(-lx) * simds / W;
+    if (g > ntile * nx) g = ntile * nx;
+  }
+  groups = (int)(g < 1 ? 1 : g);
 }
 
 template <class real>
@@ -648,11 +769,6 @@ march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx
   const int b = blockIdx.x;
   const int g = xcd_order(b, G);   // neighbouring runs on one XCD (shared L2)
   const long long ntile = (long long)nty * ntz;
-  const long long total = ntile * nx;
-  const int nfull = nx / L;                              // chunks of exactly L planes
-  const long long full_items = ntile * nfull * L;
-  long long a = (long long)g * total / G;
-  const long long e = (long long)(g + 1) * total / G;
   M m;
   m.wave = threadIdx.x / WAVE;
   m.lane = threadIdx.x % WAVE;
@@ -665,27 +781,9 @@ march3d_kernel(Fields<typename P::real> in, Fields<typename P::real> out, int nx
   } else {
     m.edge = nullptr;
   }
-  // a, e are uniform over the workgroup, so every wave runs the same segments
-  // and reaches the same barriers
-  while (a < e) {
-    int c0, len;         // first plane and length of the item's chunk
-    long long r;         // item index within the chunk
-    if (a < full_items) {
-      const long long per_chunk = ntile * L;
-      c0 = (int)(a / per_chunk) * L;
-      len = L;
-      r = a % per_chunk;
-    } else {
-      c0 = nfull * L;
-      len = nx - c0;
-      r = a - full_items;
-    }
-    const int tile = (int)(r / len);
-    const int x0 = c0 + (int)(r % len);
-    const int xend = (int)min((long long)(c0 + len), x0 + (e - a));
+  march_runs(g, G, ntile, nx, L, [&](int tile, int x0, int xend) {
     march_run<P, T, BY, W, PF>(m, in, out, nx, ny, nz, ntz, tile, x0, xend, prm);
-    a += xend - x0;
-  }
+  });
 }
 
 }  // namespace fmr

@@ -221,6 +221,14 @@ def whole_runs(occ: int) -> int:
     return -200 - occ
 
 
+def spare_runs(occ: int) -> int:
+    """The Formura_Set_March_Chunk value of the whole-run split with spare
+    workgroups: whole runs, and when they leave resident workgroups idle, the
+    runs shortened and the planes left at the end of every tile shared by the
+    idle ones (fmr::march_split_of)."""
+    return -300 - occ
+
+
 def march_chunks(occ: int) -> tuple:
     """The work splits (Formura_Set_March_Chunk values) offered for a march
     variant compiled for `occ` waves per SIMD: runs of 32, 64, 128 or 256
@@ -228,11 +236,13 @@ def march_chunks(occ: int) -> tuple:
     together, so neighbouring tiles read their shared ghost rows while they
     are in L2), balanced runs (one round of resident workgroups, one run
     each), whole runs (k runs per tile, k x tiles workgroups resident at
-    once, all in step), or x whole with one or two (four, for occ >= 4)
-    waves per SIMD.
+    once, all in step), whole runs with the idle resident workgroups taking
+    the planes at the end of every tile, or x whole with one or two (four,
+    for occ >= 4) waves per SIMD.
     The benchmark's search chooses among exactly these, and the full-size
     parity tests run every compiled variant with each of them."""
-    return (32, 64, 128, 256, balanced_runs(occ), whole_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
+    return (32, 64, 128, 256, balanced_runs(occ), whole_runs(occ), spare_runs(occ), -1, -2) + \
+        ((-4,) if occ >= 4 else ())
 
 
 def load_program(name: str) -> StepLibrary:

@@ -67,9 +67,12 @@ int Formura_Last_Timing(float *main_ms, int *main_launches, float *total_ms);
 int Formura_Set_Kernel(int kind, int T, int rows, int waves, int pf, int occ);
 
 /* Work split of the march kernel.  lx > 0: x is cut into chunks of lx
- * planes, one workgroup per (chunk, tile); lx < -200: x is cut into the
- * most chunks k per tile whose k x tiles workgroups are resident at once with
- * (-lx - 200) waves per SIMD, one workgroup per (chunk, tile) (whole runs);
+ * planes, one workgroup per (chunk, tile); -300 <= lx < -200: x is cut into
+ * the most chunks k per tile whose k x tiles workgroups are resident at once
+ * with (-lx - 200) waves per SIMD, one workgroup per (chunk, tile) (whole
+ * runs); lx < -300: whole runs with (-lx - 300) waves per SIMD, and the
+ * resident workgroups they leave idle take the last planes of every tile,
+ * the runs shortened so that no workgroup marches longer than needed;
  * -200 <= lx < -100: (-lx - 100) waves per SIMD of the device, x cut into the
  * chunks that give each workgroup one run of its own (balanced runs); other
  * lx < 0: x is kept whole and -lx waves per SIMD of the device share the

@@ -16,7 +16,7 @@ import pytest
 
 from formura_amd import build
 from formura_amd.runtime import (KIND_DIRECT, KIND_STAGED, MARCH_KINDS, StepLibrary, balanced_runs, load_program,
-                                 whole_runs)
+                                 spare_runs, whole_runs)
 from tests.common import gray_scott_ic, normal
 
 pytestmark = pytest.mark.gpu
@@ -113,9 +113,11 @@ def test_every_kernel_variant_bitwise(name, mk):
     variants = [(KIND_DIRECT, 1, 0, 1, 1, 1)] + lib.list_kernels()
     for kind, T, by, wv, pf, occ in variants:
         # runs of 64 or 7 planes (7: runs cross tile boundaries), balanced
-        # and whole runs, and one or two waves per SIMD with x whole (runs of a
-        # plane or two on these small grids)
-        splits = (64, 7, balanced_runs(occ), whole_runs(occ), -1, -2) + ((-4,) if occ >= 4 else ())
+        # and whole runs (with spare workgroups: the same split on these small
+        # grids), and one or two waves per SIMD with x whole (runs of a plane
+        # or two on these small grids)
+        splits = (64, 7, balanced_runs(occ), whole_runs(occ), spare_runs(occ), -1, -2) + \
+            ((-4,) if occ >= 4 else ())
         for lx in (splits if kind in MARCH_KINDS else (64,)):
             label = f"{name} kind={kind} T={T} rows={by} waves={wv} pf={pf} occ={occ} lx={lx}"
             _say(f"[parity] {label} steps={steps}")
